@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Benchmark: RI J+K Fock builds/sec, mixed precision vs FP64, on one MI355X.
+
+Workload (large real system): B[n_aux=4000][n=1000][n=1000] symmetric, N(0,1) exp(-|mu-nu|/50),
+generated on the device and resident in HBM in fp64, fp32 and bf16 copies; n_occ = 200 orthonormal
+orbitals from a seeded QR; a sequence of 20 further orbital sets, each rotated from the previous
+one by an angle that shrinks 10x per step (so |D_k - D_{k-1}| shrinks 10x per step).
+
+One step = one SCF-like sequence of 22 Fock builds:
+  mixed : full FP64 build at D_0, 20 incremental builds F_k = F_{k-1} + G(D_k - D_{k-1}) with bf16
+          operands on the matrix cores (fp32 accumulation), final full FP64 refinement build at D_20;
+  fp64  : the same 22 builds as full FP64 builds (reported beside it as fp64_builds_per_s).
+value = mixed builds/sec = 22 * steps / time. |dE| and max|dF| against the FP64 build at D_20 are
+reported before and after the refinement. All inputs are resident in HBM before the timed region.
+
+cpu_baseline: the oracle's FP64 restatement of the reference CPU algorithm (two GEMV passes for J,
+OpenMP static loop over the aux index with per-thread K partials, serial BLAS inside) timed on the
+host on a bounded sample of the same workload (the first n_aux_sample aux rows), scaled to n_aux.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_AUX, N, N_OCC, N_DELTA = 4000, 1000, 200, 20
+SEED = 20261203 + 1
+PEAK = {"hbm": 8000.0, "f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # GB/s, TFLOP/s (dense)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--naux", type=int, default=N_AUX)
+    p.add_argument("--n", type=int, default=N)
+    p.add_argument("--nocc", type=int, default=N_OCC)
+    p.add_argument("--delta-precision", default="bf16", choices=["bf16", "f32", "f64"])
+    p.add_argument("--cpu-sample", type=int, default=240, help="aux rows timed on the CPU (0: skip)")
+    p.add_argument("--no-fp64-leg", action="store_true")
+    return p.parse_args()
+
+
+def orbital_sequence(n, nocc, nsteps, seed):
+    from rerdmft_amd import synthetic as syn
+    g = syn.rng(seed)
+    q, _ = np.linalg.qr(g.standard_normal((n, n)))
+    cs = [np.ascontiguousarray(q[:, :nocc])]
+    angle = 0.1
+    basis = q.copy()
+    for k in range(nsteps):
+        c = syn.rotate_occ(cs[-1], basis, angle, seed + 1 + k)
+        # keep an orthonormal completion of the new occupied space for the next rotation
+        proj = basis[:, nocc:] - c @ (c.T @ basis[:, nocc:])
+        v, _ = np.linalg.qr(proj)
+        basis = np.concatenate([c, v[:, : n - nocc]], axis=1)
+        cs.append(c)
+        angle *= 0.1
+    return cs
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (the path would shard the "
+              "aux index over GPUs with one all-reduce of J and K per build; see DESIGN.md).", file=sys.stderr)
+        return 2
+    import torch
+    import rerdmft_amd as rr
+
+    dev = torch.device("cuda:0")
+    n, naux, nocc = a.n, a.naux, a.nocc
+    ctx = rr.Context(0)
+    t0 = time.time()
+    ctx.generate(0, naux, n, 1.0, 50.0, SEED, rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16)
+    cs = orbital_sequence(n, nocc, N_DELTA, SEED + 7)
+    c_dev = [torch.tensor(c, device=dev) for c in cs]
+    l_dev = [np.sqrt(2.0) * c for c in c_dev]
+    d_dev = [2.0 * (c @ c.T) for c in c_dev]
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    dprec = {"bf16": rr.BF16, "f32": rr.FP32, "f64": rr.FP64}[a.delta_precision]
+
+    J = torch.empty((n, n), dtype=torch.float64, device=dev)
+    K = torch.empty((n, n), dtype=torch.float64, device=dev)
+    dJ = torch.empty_like(J)
+    dK = torch.empty_like(K)
+
+    def full_build(k, prec=rr.FP64):
+        ctx.ri_jk_factor(0, d_dev[k], l_dev[k], precision=prec, j_out=J, k_out=K)
+        return J - 0.5 * K  # two-electron part of F (h = 0)
+
+    def mixed_sequence():
+        f = full_build(0)
+        for k in range(1, N_DELTA + 1):
+            ctx.ri_jk_delta(0, c_dev[k], c_dev[k - 1], precision=dprec, dj_out=dJ, dk_out=dK)
+            f = f + (dJ - 0.5 * dK)
+        pre = f
+        f = full_build(N_DELTA)
+        return pre, f
+
+    def fp64_sequence():
+        f = None
+        for k in range(N_DELTA + 2):
+            f = full_build(min(k, N_DELTA))
+        return f
+
+    builds = N_DELTA + 2
+    for _ in range(a.warmup):
+        mixed_sequence()
+    torch.cuda.synchronize()
+    ctx.timing(enable=True, reset=True)
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        pre, f_mixed = mixed_sequence()
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t_start
+    phases = ctx.timing(enable=False, reset=True)
+
+    # accuracy at D_20 against the FP64 build
+    f_ref = full_build(N_DELTA).clone()
+    d20 = d_dev[N_DELTA]
+    de_pre = 0.5 * float(torch.sum(d20 * (pre - f_ref)))
+    de_fin = 0.5 * float(torch.sum(d20 * (f_mixed - f_ref)))
+    df_pre = float(torch.max(torch.abs(pre - f_ref)))
+    df_fin = float(torch.max(torch.abs(f_mixed - f_ref)))
+    e_ref = 0.5 * float(torch.sum(d20 * f_ref))
+
+    fp64_rate = None
+    fp64_phases = None
+    if not a.no_fp64_leg:
+        fp64_sequence()
+        torch.cuda.synchronize()
+        ctx.timing(enable=True, reset=True)
+        t1 = time.perf_counter()
+        fp64_sequence()
+        torch.cuda.synchronize()
+        t_fp64 = time.perf_counter() - t1
+        fp64_phases = ctx.timing(enable=False, reset=True)
+        fp64_rate = builds / t_fp64
+
+    rate = builds * a.steps / t_mixed
+
+    # roofline of the dominant kernel in the mixed sequence (phase times are per-launch averages
+    # from HIP events on the library stream)
+    name, (tot_ms, cnt) = max(phases.items(), key=lambda kv: kv[1][0])
+    flop_half_full = 2.0 * naux * n * n * nocc          # exchange half-transform, full build (r = n_occ)
+    flop_half_delta = 2.0 * naux * n * n * 2 * nocc     # incremental build (r = 2 n_occ)
+    flop_rank_full = 1.0 * naux * n * n * nocc          # Hermitian rank-k (lower triangle)
+    flop_rank_delta = 2.0 * naux * n * n * nocc         # full G = W_M W_dC^T
+    bytes_stream = {rr.FP64: 8, rr.FP32: 4, rr.BF16: 2}
+    n_full = 2 * a.steps
+    n_delta = N_DELTA * a.steps
+    if name in ("exchange_half", "exchange_rank_k"):
+        if name == "exchange_half":
+            work = n_full * flop_half_full + n_delta * flop_half_delta
+        else:
+            work = n_full * flop_rank_full + n_delta * flop_rank_delta
+        achieved = work / (tot_ms * 1e-3) / 1e12
+        # peak: time-weighted between the fp64 launches and the low-precision launches
+        peak = PEAK["bf16"] if dprec == rr.BF16 else (PEAK["f32"] if dprec == rr.FP32 else PEAK["f64"])
+        roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
+    else:
+        per = naux * n * n
+        work = n_full * per * 8 + n_delta * per * bytes_stream[dprec]
+        achieved = work / (tot_ms * 1e-3) / 1e9
+        roof = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK["hbm"], "unit": "GB/s",
+                "frac": achieved / PEAK["hbm"], "traffic": None}
+
+    # per-kernel rooflines of the FP64 build (for the record)
+    fp64_kernels = {}
+    if fp64_phases:
+        nb = builds
+        for k2, (ms2, c2) in fp64_phases.items():
+            if c2 == 0:
+                continue
+            avg = ms2 / c2
+            if k2 in ("gamma", "coulomb"):
+                fp64_kernels[k2] = {"avg_ms": avg, "GB/s": naux * n * n * 8 / (avg * 1e-3) / 1e9}
+            elif k2 == "exchange_half":
+                fp64_kernels[k2] = {"avg_ms": avg, "TFLOP/s": flop_half_full / (avg * 1e-3) / 1e12}
+            elif k2 == "exchange_rank_k":
+                fp64_kernels[k2] = {"avg_ms": avg, "TFLOP/s(alg, full K)": 2 * flop_rank_full / (avg * 1e-3) / 1e12}
+            else:
+                fp64_kernels[k2] = {"avg_ms": avg}
+
+    cpu = None
+    if a.cpu_sample > 0:
+        cpu = cpu_baseline(n, nocc, min(a.cpu_sample, naux), naux, cs[0])
+
+    out = {
+        "metric": "RI J+K Fock builds/sec (mixed vs FP64) at |dE|<=1e-8 Ha; % HBM/MFMA peak",
+        "value": rate,
+        "unit": "Fock builds/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_mixed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{a.delta_precision}+f64",
+        "data": "synthetic",
+        "config": {"workload": "real RI J/K, B[4000][1000][1000] sym N(0,1)exp(-|mu-nu|/50), n_occ=200, "
+                               "20 incremental builds (|dD| /10 per step) + FP64 refinement",
+                   "n_aux": naux, "n": n, "n_occ": nocc, "builds_per_step": builds,
+                   "delta_precision": a.delta_precision},
+        "fp64_builds_per_s": fp64_rate,
+        "speedup_vs_fp64": (rate / fp64_rate) if fp64_rate else None,
+        "dE_after_refinement": de_fin,
+        "dF_after_refinement": df_fin,
+        "dE_before_refinement": de_pre,
+        "dF_before_refinement": df_pre,
+        "E_two_electron": e_ref,
+        "roofline": roof,
+        "fp64_build_kernels": fp64_kernels,
+        "phase_ms_mixed": {k: v[0] for k, v in phases.items()},
+        "cpu_baseline": cpu,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+def cpu_baseline(n, nocc, nsample, naux, c0):
+    """Oracle timed on the host: J + K over `nsample` aux rows of the same distribution, scaled."""
+    from oracle import oracle as orc
+    from rerdmft_amd import synthetic as syn
+    orc.lib()
+    b = np.empty((nsample, n * n))
+    g = syn.rng(SEED)
+    dec = syn.decay_matrix(n, 50.0).ravel()
+    for p in range(nsample):
+        z = g.standard_normal((n, n))
+        z = np.triu(z) + np.triu(z, 1).T
+        b[p] = z.ravel() * dec
+    d = 2.0 * c0 @ c0.T
+    l = np.sqrt(2.0) * c0
+    orc.ri_jk_factor(b[:8], d, l)  # warm-up (thread pool, MKL dispatch)
+    t0 = time.perf_counter()
+    orc.ri_jk_factor(b, d, l)
+    dt = time.perf_counter() - t0
+    per_build = dt * naux / nsample
+    # the reference additionally recovers C_occ from D each build (FP64 eigensolve, n x n)
+    x = np.eye(n)
+    t1 = time.perf_counter()
+    orc.nonrel_fock_ri(b[:1], x, x, np.zeros((n, n)), d)
+    t_rec = time.perf_counter() - t1
+    return {"value": 1.0 / per_build, "unit": "Fock builds/s", "cores": orc.max_threads(), "kind": "port",
+            "sample": f"J+K over the first {nsample} of {naux} aux rows at n={n}, n_occ={nocc} "
+                      f"({dt:.2f} s), scaled by {naux}/{nsample}; occupied recovery excluded",
+            "value_with_recovery": 1.0 / (per_build + t_rec), "recovery_s": t_rec,
+            "blas": "mkl" if orc.has_mkl() else "portable"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,144 @@
+/* librerdmft_gpu -- MI355X (gfx950) RI / Cholesky J/K Fock builds and RI AO->MO transforms.
+ *
+ * A C ABI over plain pointers and sizes. Every array is row-major, float64 or complex128 stored as
+ * interleaved (re, im) doubles, and may live in host memory or in device memory of the handle's GPU
+ * (the library detects which). Calls are synchronous: outputs are complete when a call returns.
+ * Errors are reported as non-zero status codes; rr_last_error() returns the message, which for
+ * shape errors is the message the reference throws as std::runtime_error.
+ *
+ * Reference call sites each entry point replaces are cited as file:line of marm314/ReRDMFT.
+ */
+#ifndef RERDMFT_GPU_H
+#define RERDMFT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rr_context* rr_handle;
+
+/* status codes */
+#define RR_OK 0
+#define RR_ERR_ARG 1      /* invalid argument */
+#define RR_ERR_DIM 2      /* dimension mismatch (reference: std::runtime_error on shape checks) */
+#define RR_ERR_HIP 3      /* HIP runtime error */
+#define RR_ERR_SOLVER 4   /* eigensolver failure (reference: LAPACK non-convergence) */
+#define RR_ERR_STATE 5    /* missing tensor slot / precision copy */
+#define RR_ERR_INTERNAL 6
+
+/* arithmetic of a contraction: operands stored in this type, matrix cores accumulate in
+ * fp64 (RR_FP64) or fp32 (RR_FP32, RR_BF16) */
+#define RR_FP64 0
+#define RR_FP32 1
+#define RR_BF16 2
+
+/* copies of a 3-index tensor kept resident in HBM */
+#define RR_COPY_FP64 1u
+#define RR_COPY_FP32 2u
+#define RR_COPY_BF16 4u
+
+/* ---- context ---------------------------------------------------------------------------- */
+int rr_create(int device, rr_handle* out);
+int rr_destroy(rr_handle h);
+const char* rr_last_error(rr_handle h);
+/* Select a GEMM tile configuration for every launch (-1: per-shape autotuned choice). Results do
+ * not depend on the choice. */
+int rr_set_tile_config(rr_handle h, int cfg);
+/* Per-phase device time (HIP events on the library stream) accumulated since the last reset:
+ * ms[0] gamma stream, ms[1] Coulomb stream, ms[2] exchange half-transform GEMM, ms[3] exchange
+ * rank-k GEMM, ms[4] assembly, ms[5] recovery; counts[i] = launches timed. enable=0 turns the
+ * event timing off (default off). */
+int rr_timing(rr_handle h, int enable, int reset, double* ms, long long* counts, int n);
+
+/* ---- 3-index tensors (AO_ints/ThreeCenterIntegrals.h:42-70 layout: (n_aux, n*n), row P is the
+ * n x n block (mu nu|P)); also the Cholesky vectors of Utils/AoCholesky.h:23-41 ------------- */
+int rr_tensor_upload(rr_handle h, int slot, const double* eri3, size_t n_aux, size_t n, unsigned copies);
+/* Synthetic symmetric tensor generated on the device: sigma * N(0,1) * exp(-|mu-nu|/decay). */
+int rr_tensor_generate(rr_handle h, int slot, size_t n_aux, size_t n, double sigma, double decay, uint64_t seed,
+                       unsigned copies);
+int rr_tensor_info(rr_handle h, int slot, size_t* n_aux, size_t* n, unsigned* copies);
+int rr_tensor_release(rr_handle h, int slot);
+
+/* ---- NON_REL ------------------------------------------------------------------------------ */
+/* nonRelFockMatrix(h, RiNonRelEriSource, P): src/NON_REL/NonRelFockMatrixRi.cpp:27-100
+ * (declared NonRelFockMatrixRi.h:39-40). F = h + J - K/2 with the reference's occupied-orbital
+ * recovery from P (eigenvalues of X^T (S P S) X / 2 above 0.5). j_out / k_out may be NULL. */
+int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const double* h_core, const double* x_large,
+                      size_t nx, const double* overlap, const double* density, double* fock, double* j_out,
+                      double* k_out);
+
+/* Factor-form real J/K without recovery (the exchange loop of NonRelFockMatrixRi.cpp:74-94 with a
+ * caller-supplied factor): J = sum_P tr(B_P D) B_P, K = sum_P B_P^T (L R^T) B_P with L, R n x r
+ * (rfac NULL: R = L). Either of j_out / k_out may be NULL. */
+int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double* density, const double* lfac,
+                    const double* rfac, size_t r, double* j_out, double* k_out);
+
+/* Incremental build for D = 2 C C^T (no reference counterpart; replaces a full call of
+ * NonRelFockMatrixRi.cpp:27-100 between two SCF iterations): dJ = J(D_new - D_old),
+ * dK = K(D_new - D_old) from the aligned-difference factorization of D_new - D_old, so the
+ * low-precision error is proportional to |D_new - D_old|. */
+int rr_ri_jk_delta(rr_handle h, int slot, int precision, size_t n, size_t n_occ, const double* c_new,
+                   const double* c_old, double* dj_out, double* dk_out);
+
+/* ---- X2C ---------------------------------------------------------------------------------- */
+/* x2cFockMatrix(h, RiX2cEriSource, P): src/X2C_DHF/X2C_FockMatrixRi.cpp:35-162 (declared
+ * X2C_FockMatrixRi.h:38-40). Complex 2nl x 2nl arrays; x_large is 2nl x nx. */
+int rr_x2c_fock_ri(rr_handle h, int slot, int precision, size_t nl, const double* h_x2c, const double* x_large,
+                   size_t nx, const double* overlap, const double* density, double* fock);
+
+/* ---- 4c RKB (UKB blocks) ------------------------------------------------------------------- */
+/* riFockTwoElectronDirect: src/UKB/UkbFockMatrixRi.cpp:35-183 (declared UkbFockMatrixRi.h:70-75).
+ * slot_ll / slot_ss hold eri3_LL (n_aux, nl^2) and eri3_SS (n_aux, ns^2) sharing one aux index. */
+int rr_ri_fock_two_electron_direct(rr_handle h, int slot_ll, int slot_ss, int precision, size_t nl, size_t ns,
+                                   const double* density_ukb, const double* c_occ_ukb, size_t n_occ,
+                                   const double* occupations, double* g_out);
+/* rkbFockMatrix(h, RiDirectEriSource, P): src/UKB/UkbFockMatrixRi.cpp:185-216 (declared
+ * UkbFockMatrixRi.h:96-98). v_total is n_ukb x n_rkb, s_full n_rkb x n_rkb, x_full n_rkb x nx. */
+int rr_rkb_fock_ri(rr_handle h, int slot_ll, int slot_ss, int precision, size_t nl, size_t ns, const double* h_rkb,
+                   const double* v_total, size_t n_rkb, const double* s_full, const double* x_full, size_t nx,
+                   const double* density, double* fock);
+
+/* ---- MO-basis RI tensors ------------------------------------------------------------------- */
+/* buildRiMoEri: src/UKB/RiMoEri.cpp:64-86 (+ transformBlock :20-62). c_ukb is n_ukb x n_total
+ * complex; out is (n_aux, n_total^2) complex, B(P,p,q) = sum conj(C(mu,p)) C(nu,q) (mu nu|P).
+ * ns = 0 gives the X2C 2-block transform with slot_ss unused. out_precision RR_FP64 writes
+ * complex128, RR_FP32 writes complex64 (interleaved floats). */
+int rr_build_ri_mo_eri(rr_handle h, int slot_ll, int slot_ss, int precision, int out_precision, size_t nl, size_t ns,
+                       const double* c_ukb, size_t n_total, void* out);
+/* RiMoEri::rotated: src/UKB/RiMoEri.cpp:88-110. B'_P = U^H B_P U for b (n_aux, n^2) complex. */
+int rr_ri_mo_eri_rotated(rr_handle h, int precision, size_t n_total, size_t n_aux, const double* b, const double* u,
+                         double* out);
+/* RiMoEri::positiveBlock: src/UKB/RiMoEri.cpp:112-121. */
+int rr_ri_mo_eri_positive_block(rr_handle h, size_t n_total, size_t n_aux, const double* b, size_t off, double* out);
+/* buildRiNonRelSpinMoEri: src/NON_REL/NonRelSpinRiMoEri.cpp:13-43. B^MO_P = C^T B_P C (real);
+ * c_spatial is n x n_mo (n_mo = n in the reference). out (n_aux, n_mo^2). */
+int rr_build_ri_nonrel_spin_mo_eri(rr_handle h, int slot, int precision, size_t n, const double* c_spatial,
+                                   size_t n_mo, double* out);
+/* RiNonRelSpinMoEri::rotated: src/NON_REL/NonRelSpinRiMoEri.cpp:45-95 (u is 2n x 2n). */
+int rr_ri_nonrel_spin_mo_eri_rotated(rr_handle h, int precision, size_t n_spatial, size_t n_aux, const double* b,
+                                     const double* u, double* out);
+/* RiNonRelSpinMoEri::embeddedB: src/NON_REL/NonRelSpinRiMoEri.h:77-89. */
+int rr_ri_nonrel_spin_embedded_b(rr_handle h, size_t n_spatial, size_t n_aux, const double* b, double* out);
+
+/* ---- Cholesky vectors (slot uploaded from AoCholesky::vectors, (n_chol, n*n)) ---------------- */
+/* nonRelFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:108-149 (K = sum_L B_L P B_L). */
+int rr_nonrel_fock_cholesky(rr_handle h, int slot, int precision, size_t n, const double* h_core, const double* p,
+                            double* fock);
+/* x2cFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:151-217. */
+int rr_x2c_fock_cholesky(rr_handle h, int slot, int precision, size_t n, const double* h_x2c, const double* p,
+                         double* fock);
+/* aoCholeskyToMoSpinOrbital (spatial block C^T B_L C): src/Utils/AoCholesky.cpp:219-243. */
+int rr_cholesky_to_mo_spatial(rr_handle h, int slot, int precision, size_t n, const double* c, size_t n_mo,
+                              double* out);
+/* aoCholeskyToMoSpinor: src/Utils/AoCholesky.cpp:270-295 (out W_L = (sum_s C_s^H B_L C_s)^T). */
+int rr_cholesky_to_mo_spinor(rr_handle h, int slot, int precision, size_t n, const double* c, size_t n_mo,
+                             double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RERDMFT_GPU_H */

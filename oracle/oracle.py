@@ -1,0 +1,204 @@
+"""ctypes front-end of the CPU oracle (oracle/rr_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the CPU-baseline leg of
+bench.py, always as the checker. The GPU library (rerdmft_amd) never imports this module.
+
+Arrays are numpy, row-major, float64 or complex128 (complex passed as interleaved (re, im)).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_sz = ctypes.c_size_t
+_dp = ctypes.c_void_p
+
+
+def build():
+    """Compile the oracle with its Makefile (g++ + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.oracle_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+
+
+def has_mkl():
+    return bool(lib().oracle_has_mkl())
+
+
+def max_threads():
+    return int(lib().oracle_max_threads())
+
+
+def nonrel_fock_ri(eri3, x_large, overlap, h_core, density, return_jk=False):
+    """nonRelFockMatrix(h, RiNonRelEriSource, P) -- src/NON_REL/NonRelFockMatrixRi.cpp:27-100."""
+    eri3, x, s, h, d = map(_f64, (eri3, x_large, overlap, h_core, density))
+    naux, n2 = eri3.shape
+    n = h.shape[0]
+    assert n * n == n2
+    f = np.empty((n, n))
+    j = np.empty((n, n))
+    k = np.empty((n, n))
+    _check(lib().oracle_nonrel_fock_ri(_sz(n), _sz(naux), _p(eri3), _p(x), _sz(x.shape[1]), _p(s), _p(h), _p(d),
+                                       _p(f), _p(j), _p(k)))
+    return (f, j, k) if return_jk else f
+
+
+def ri_jk_factor(eri3, d, lfac, rfac=None):
+    """J = sum_P tr(B_P D) B_P and K = sum_P B_P^T (L R^T) B_P (closed form, no recovery)."""
+    eri3, d, lfac = map(_f64, (eri3, d, lfac))
+    rfac = lfac if rfac is None else _f64(rfac)
+    naux, n2 = eri3.shape
+    n = d.shape[0]
+    j = np.empty((n, n))
+    k = np.empty((n, n))
+    _check(lib().oracle_ri_jk_factor(_sz(n), _sz(naux), _p(eri3), _p(d), _p(lfac), _p(rfac), _sz(lfac.shape[1]),
+                                     _p(j), _p(k)))
+    return j, k
+
+
+def x2c_fock_ri(eri3, x_large, overlap, h, density):
+    """x2cFockMatrix(h, RiX2cEriSource, P) -- src/X2C_DHF/X2C_FockMatrixRi.cpp:35-162."""
+    eri3 = _f64(eri3)
+    x, s, h, d = map(_c128, (x_large, overlap, h, density))
+    naux = eri3.shape[0]
+    nl = h.shape[0] // 2
+    f = np.empty_like(h)
+    _check(lib().oracle_x2c_fock_ri(_sz(nl), _sz(naux), _p(eri3), _p(x), _sz(x.shape[1]), _p(s), _p(h), _p(d), _p(f)))
+    return f
+
+
+def ri_fock_two_electron_direct(nl, ns, eri3_ll, eri3_ss, density_ukb, c_occ_ukb, occupations):
+    """riFockTwoElectronDirect -- src/UKB/UkbFockMatrixRi.cpp:35-183."""
+    ell, ess = _f64(eri3_ll), _f64(eri3_ss)
+    d, c = _c128(density_ukb), _c128(c_occ_ukb)
+    occ = _f64(occupations)
+    n = 2 * nl + 2 * ns
+    g = np.empty((n, n), dtype=np.complex128)
+    _check(lib().oracle_ri_fock_two_electron_direct(_sz(nl), _sz(ns), _sz(ell.shape[0]), _p(ell), _p(ess), _p(d), _p(c),
+                                                    _sz(occ.shape[0]), _p(occ), _p(g)))
+    return g
+
+
+def rkb_fock_ri(nl, ns, eri3_ll, eri3_ss, v_total, s_full, x_full, h_rkb, density):
+    """rkbFockMatrix(h, RiDirectEriSource, P) -- src/UKB/UkbFockMatrixRi.cpp:185-216."""
+    ell, ess = _f64(eri3_ll), _f64(eri3_ss)
+    v, s, x, h, d = map(_c128, (v_total, s_full, x_full, h_rkb, density))
+    n_rkb = h.shape[0]
+    f = np.empty_like(h)
+    _check(lib().oracle_rkb_fock_ri(_sz(nl), _sz(ns), _sz(ell.shape[0]), _p(ell), _p(ess), _p(v), _sz(n_rkb), _p(s),
+                                    _p(x), _sz(x.shape[1]), _p(h), _p(d), _p(f)))
+    return f
+
+
+def build_ri_mo_eri(nl, ns, eri3_ll, eri3_ss, c_ukb):
+    """buildRiMoEri -- src/UKB/RiMoEri.cpp:64-86. Returns (naux, n_total, n_total) complex."""
+    ell, ess, c = _f64(eri3_ll), _f64(eri3_ss), _c128(c_ukb)
+    naux = ell.shape[0]
+    nt = c.shape[1]
+    out = np.empty((naux, nt, nt), dtype=np.complex128)
+    _check(lib().oracle_build_ri_mo_eri(_sz(nl), _sz(ns), _sz(naux), _p(ell), _p(ess), _p(c), _sz(nt), _p(out)))
+    return out
+
+
+def ri_mo_eri_rotated(b, u):
+    """RiMoEri::rotated -- src/UKB/RiMoEri.cpp:88-110."""
+    b, u = _c128(b), _c128(u)
+    naux, nt = b.shape[0], u.shape[0]
+    out = np.empty((naux, nt, nt), dtype=np.complex128)
+    _check(lib().oracle_ri_mo_eri_rotated(_sz(nt), _sz(naux), _p(b), _p(u), _p(out)))
+    return out
+
+
+def build_ri_nonrel_spin_mo_eri(eri3, c_spatial):
+    """buildRiNonRelSpinMoEri -- src/NON_REL/NonRelSpinRiMoEri.cpp:13-43. (naux, n, n) real."""
+    eri3, c = _f64(eri3), _f64(c_spatial)
+    naux, n = eri3.shape[0], c.shape[0]
+    out = np.empty((naux, n, n))
+    _check(lib().oracle_build_ri_nonrel_spin_mo_eri(_sz(n), _sz(naux), _p(eri3), _p(c), _p(out)))
+    return out
+
+
+def ri_nonrel_spin_mo_eri_rotated(b, u):
+    """RiNonRelSpinMoEri::rotated -- src/NON_REL/NonRelSpinRiMoEri.cpp:45-95."""
+    b, u = _f64(b), _f64(u)
+    naux, n = b.shape[0], u.shape[0] // 2
+    out = np.empty((naux, n, n))
+    _check(lib().oracle_ri_nonrel_spin_mo_eri_rotated(_sz(n), _sz(naux), _p(b), _p(u), _p(out)))
+    return out
+
+
+def nonrel_fock_cholesky(vectors, h, p):
+    """nonRelFockMatrix(h, AoCholesky, P) -- src/Utils/AoCholesky.cpp:108-149."""
+    v, h, p = _f64(vectors), _f64(h), _f64(p)
+    n = h.shape[0]
+    f = np.empty((n, n))
+    _check(lib().oracle_nonrel_fock_cholesky(_sz(n), _sz(v.shape[0]), _p(v), _p(h), _p(p), _p(f)))
+    return f
+
+
+def x2c_fock_cholesky(vectors, h, p):
+    """x2cFockMatrix(h, AoCholesky, P) -- src/Utils/AoCholesky.cpp:151-217."""
+    v = _f64(vectors)
+    h, p = _c128(h), _c128(p)
+    n = h.shape[0] // 2
+    f = np.empty_like(h)
+    _check(lib().oracle_x2c_fock_cholesky(_sz(n), _sz(v.shape[0]), _p(v), _p(h), _p(p), _p(f)))
+    return f
+
+
+def cholesky_to_mo_spatial(vectors, c):
+    """Spatial block of aoCholeskyToMoSpinOrbital -- src/Utils/AoCholesky.cpp:219-243."""
+    v, c = _f64(vectors), _f64(c)
+    nchol, n, nmo = v.shape[0], c.shape[0], c.shape[1]
+    out = np.empty((nchol, nmo, nmo))
+    _check(lib().oracle_cholesky_to_mo_spatial(_sz(n), _sz(nchol), _p(v), _p(c), _sz(nmo), _p(out)))
+    return out
+
+
+def cholesky_to_mo_spinor(vectors, c):
+    """aoCholeskyToMoSpinor -- src/Utils/AoCholesky.cpp:270-295."""
+    v, c = _f64(vectors), _c128(c)
+    nchol, nmo = v.shape[0], c.shape[1]
+    n = c.shape[0] // 2
+    out = np.empty((nchol, nmo, nmo), dtype=np.complex128)
+    _check(lib().oracle_cholesky_to_mo_spinor(_sz(n), _sz(nchol), _p(v), _p(c), _sz(nmo), _p(out)))
+    return out
+
+
+def eig_hermitian(a):
+    a = _c128(a)
+    n = a.shape[0]
+    w = np.empty(n)
+    v = np.empty((n, n), dtype=np.complex128)
+    _check(lib().oracle_eig_hermitian(_sz(n), _p(a), _p(w), _p(v)))
+    return w, v

@@ -1,0 +1,808 @@
+// CPU oracle: an FP64 C++ restatement of the reference's RI / Cholesky J/K Fock builds and of its
+// RI AO->MO transforms. TEST INFRASTRUCTURE ONLY -- the tests, the smoke check and the CPU-baseline
+// leg of the benchmark load it as the checker; the GPU library never links or calls it.
+//
+// Each exported function names the reference routine it restates (file:line of marm314/ReRDMFT).
+// Loop structure follows the reference: Coulomb as two GEMV passes, exchange as an OpenMP
+// `schedule(static)` loop over the aux index with per-thread partial sums merged in thread order,
+// BLAS serial inside the parallel region (src/Utils/BlasThreads.h:9-41).
+//
+// BLAS/LAPACK: MKL's single dynamic library is loaded at run time (dlopen) when present, with its
+// GNU-OpenMP threading layer so that calls made inside our parallel regions run serially; when it
+// is absent, small portable loops and a cyclic Jacobi eigensolver stand in (fine at test sizes).
+#include <dlfcn.h>
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using C = std::complex<double>;
+thread_local std::string g_err;
+
+// Mutation switch for the oracle's own tests: RR_ORACLE_PLANT=<name> plants one named mistake so
+// the tests can show their checks reject it. Unset in every normal run.
+bool planted(const char* name) {
+  const char* e = std::getenv("RR_ORACLE_PLANT");
+  return e && std::strcmp(e, name) == 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// BLAS / LAPACK access
+// ------------------------------------------------------------------------------------------------
+enum { kRowMajor = 101, kNoTrans = 111, kTrans = 112, kConjTrans = 113 };
+using dgemm_t = void (*)(int, int, int, int, int, int, double, const double*, int, const double*, int, double, double*,
+                         int);
+using zgemm_t = void (*)(int, int, int, int, int, int, const void*, const void*, int, const void*, int, const void*,
+                         void*, int);
+using dgemv_t = void (*)(int, int, int, int, double, const double*, int, const double*, int, double, double*, int);
+using dsyev_t = int (*)(int, char, char, int, double*, int, double*);
+using zheev_t = int (*)(int, char, char, int, void*, int, double*);
+using setlayer_t = int (*)(int);
+
+struct Blas {
+  dgemm_t dgemm = nullptr;
+  zgemm_t zgemm = nullptr;
+  dgemv_t dgemv = nullptr;
+  dsyev_t dsyev = nullptr;
+  zheev_t zheev = nullptr;
+  bool mkl = false;
+  Blas() {
+    if (const char* off = std::getenv("RR_ORACLE_NO_MKL"); off && off[0] == '1') return;
+    void* h = dlopen("libmkl_rt.so.1", RTLD_NOW | RTLD_GLOBAL);
+    if (!h) h = dlopen("/opt/conda/lib/libmkl_rt.so.1", RTLD_NOW | RTLD_GLOBAL);
+    if (!h) return;
+    auto layer = reinterpret_cast<setlayer_t>(dlsym(h, "MKL_Set_Threading_Layer"));
+    if (layer) layer(3 /* MKL_THREADING_GNU */);
+    dgemm = reinterpret_cast<dgemm_t>(dlsym(h, "cblas_dgemm"));
+    zgemm = reinterpret_cast<zgemm_t>(dlsym(h, "cblas_zgemm"));
+    dgemv = reinterpret_cast<dgemv_t>(dlsym(h, "cblas_dgemv"));
+    dsyev = reinterpret_cast<dsyev_t>(dlsym(h, "LAPACKE_dsyev"));
+    zheev = reinterpret_cast<zheev_t>(dlsym(h, "LAPACKE_zheev"));
+    mkl = dgemm && zgemm && dgemv && dsyev && zheev;
+    if (!mkl) dgemm = nullptr, zgemm = nullptr, dgemv = nullptr, dsyev = nullptr, zheev = nullptr;
+  }
+};
+const Blas& blas() {
+  static Blas b;
+  return b;
+}
+
+// Row-major C = alpha op(A) op(B) + beta C.
+void gemm(bool ta, bool tb, std::size_t m, std::size_t n, std::size_t k, double alpha, const double* a, std::size_t lda,
+          const double* b, std::size_t ldb, double beta, double* c, std::size_t ldc) {
+  if (m == 0 || n == 0) return;
+  if (blas().mkl) {
+    blas().dgemm(kRowMajor, ta ? kTrans : kNoTrans, tb ? kTrans : kNoTrans, (int)m, (int)n, (int)k, alpha, a, (int)lda,
+                 b, (int)ldb, beta, c, (int)ldc);
+    return;
+  }
+  std::vector<double> row(n);
+  for (std::size_t i = 0; i < m; ++i) {
+    std::fill(row.begin(), row.end(), 0.0);
+    for (std::size_t p = 0; p < k; ++p) {
+      const double av = ta ? a[p * lda + i] : a[i * lda + p];
+      if (av == 0.0) continue;
+      for (std::size_t j = 0; j < n; ++j) row[j] += av * (tb ? b[j * ldb + p] : b[p * ldb + j]);
+    }
+    for (std::size_t j = 0; j < n; ++j) c[i * ldc + j] = alpha * row[j] + (beta == 0.0 ? 0.0 : beta * c[i * ldc + j]);
+  }
+}
+
+// Row-major complex C = op(A) op(B) + beta C, op in {N, T, H}.
+void zgemm(char ta, char tb, std::size_t m, std::size_t n, std::size_t k, const C* a, std::size_t lda, const C* b,
+           std::size_t ldb, C beta, C* c, std::size_t ldc) {
+  if (m == 0 || n == 0) return;
+  auto code = [](char t) { return t == 'N' ? kNoTrans : (t == 'T' ? kTrans : kConjTrans); };
+  if (blas().mkl) {
+    const C alpha(1.0, 0.0);
+    blas().zgemm(kRowMajor, code(ta), code(tb), (int)m, (int)n, (int)k, &alpha, a, (int)lda, b, (int)ldb, &beta, c,
+                 (int)ldc);
+    return;
+  }
+  auto ga = [&](std::size_t i, std::size_t p) {
+    return ta == 'N' ? a[i * lda + p] : (ta == 'T' ? a[p * lda + i] : std::conj(a[p * lda + i]));
+  };
+  auto gb = [&](std::size_t p, std::size_t j) {
+    return tb == 'N' ? b[p * ldb + j] : (tb == 'T' ? b[j * ldb + p] : std::conj(b[j * ldb + p]));
+  };
+  std::vector<C> row(n);
+  for (std::size_t i = 0; i < m; ++i) {
+    std::fill(row.begin(), row.end(), C{});
+    for (std::size_t p = 0; p < k; ++p) {
+      const C av = ga(i, p);
+      for (std::size_t j = 0; j < n; ++j) row[j] += av * gb(p, j);
+    }
+    for (std::size_t j = 0; j < n; ++j) c[i * ldc + j] = row[j] + beta * c[i * ldc + j];
+  }
+}
+
+// y = op(A) x, A row-major m x n.
+void gemv(bool trans, std::size_t m, std::size_t n, const double* a, const double* x, double* y) {
+  if (blas().mkl) {
+    blas().dgemv(kRowMajor, trans ? kTrans : kNoTrans, (int)m, (int)n, 1.0, a, (int)n, x, 1, 0.0, y, 1);
+    return;
+  }
+  if (!trans) {
+#pragma omp parallel for schedule(static)
+    for (std::size_t i = 0; i < m; ++i) {
+      double s = 0.0;
+      for (std::size_t j = 0; j < n; ++j) s += a[i * n + j] * x[j];
+      y[i] = s;
+    }
+  } else {
+#pragma omp parallel for schedule(static)
+    for (std::size_t j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (std::size_t i = 0; i < m; ++i) s += a[i * n + j] * x[i];
+      y[j] = s;
+    }
+  }
+}
+
+// Cyclic Jacobi for a Hermitian matrix (row-major), eigenvalues ascending, eigenvectors in columns.
+void jacobiHermitian(std::size_t n, std::vector<C>& a, std::vector<double>& w, std::vector<C>& v) {
+  v.assign(n * n, C{});
+  for (std::size_t i = 0; i < n; ++i) v[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (std::size_t p = 0; p < n; ++p)
+      for (std::size_t q = p + 1; q < n; ++q) off += std::norm(a[p * n + q]);
+    if (off < 1e-30) break;
+    for (std::size_t p = 0; p < n; ++p)
+      for (std::size_t q = p + 1; q < n; ++q) {
+        const C apq = a[p * n + q];
+        const double mag = std::abs(apq);
+        if (mag < 1e-300) continue;
+        const C ph = apq / mag;  // a_pq = |a_pq| e^{i phi}
+        const double app = a[p * n + p].real(), aqq = a[q * n + q].real();
+        const double theta = 0.5 * std::atan2(2.0 * mag, aqq - app);
+        const double c = std::cos(theta), s = std::sin(theta);
+        // Rotation R: col p' = c e_p - s conj(ph) e_q ; col q' = s ph e_p + c e_q  (unitary)
+        for (std::size_t k = 0; k < n; ++k) {  // A <- A R
+          const C akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * std::conj(ph) * akq;
+          a[k * n + q] = s * ph * akp + c * akq;
+        }
+        for (std::size_t k = 0; k < n; ++k) {  // A <- R^H A
+          const C apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * ph * aqk;
+          a[q * n + k] = s * std::conj(ph) * apk + c * aqk;
+        }
+        for (std::size_t k = 0; k < n; ++k) {  // V <- V R
+          const C vkp = v[k * n + p], vkq = v[k * n + q];
+          v[k * n + p] = c * vkp - s * std::conj(ph) * vkq;
+          v[k * n + q] = s * ph * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<std::size_t> idx(n);
+  for (std::size_t i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](std::size_t x, std::size_t y) { return a[x * n + x].real() < a[y * n + y].real(); });
+  w.resize(n);
+  std::vector<C> vs(n * n);
+  for (std::size_t j = 0; j < n; ++j) {
+    w[j] = a[idx[j] * n + idx[j]].real();
+    for (std::size_t i = 0; i < n; ++i) vs[i * n + j] = v[i * n + idx[j]];
+  }
+  v.swap(vs);
+}
+
+// Symmetric eigensolve (src/Linear_Algebra/LinearAlgebra.cpp:50-82: LAPACKE_dsyev 'V','U').
+void eigSym(std::size_t n, const double* a_in, std::vector<double>& w, std::vector<double>& v) {
+  v.assign(a_in, a_in + n * n);
+  w.assign(n, 0.0);
+  if (n == 0) return;
+  if (blas().mkl) {
+    if (blas().dsyev(kRowMajor, 'V', 'U', (int)n, v.data(), (int)n, w.data()) != 0)
+      throw std::runtime_error("diagonalizeSymmetric: LAPACKE_dsyev failed to converge");
+    return;
+  }
+  std::vector<C> a(n * n), vc;
+  for (std::size_t i = 0; i < n * n; ++i) a[i] = a_in[i];
+  jacobiHermitian(n, a, w, vc);
+  for (std::size_t i = 0; i < n * n; ++i) v[i] = vc[i].real();
+}
+
+// Hermitian eigensolve (LinearAlgebra.cpp:125-150: LAPACKE_zheev 'V','U').
+void eigHerm(std::size_t n, const C* a_in, std::vector<double>& w, std::vector<C>& v) {
+  v.assign(a_in, a_in + n * n);
+  w.assign(n, 0.0);
+  if (n == 0) return;
+  if (blas().mkl) {
+    if (blas().zheev(kRowMajor, 'V', 'U', (int)n, v.data(), (int)n, w.data()) != 0)
+      throw std::runtime_error("diagonalizeHermitian: LAPACKE_zheev failed to converge");
+    return;
+  }
+  std::vector<C> a(a_in, a_in + n * n);
+  jacobiHermitian(n, a, w, v);
+}
+
+std::vector<double> dmul(std::size_t m, std::size_t k, std::size_t n, const double* a, const double* b, bool ta = false,
+                         bool tb = false) {
+  std::vector<double> c(m * n, 0.0);
+  gemm(ta, tb, m, n, k, 1.0, a, ta ? m : k, b, tb ? k : n, 0.0, c.data(), n);
+  return c;
+}
+std::vector<C> zmul(std::size_t m, std::size_t k, std::size_t n, const C* a, const C* b, char ta = 'N', char tb = 'N') {
+  std::vector<C> c(m * n, C{});
+  zgemm(ta, tb, m, n, k, a, ta == 'N' ? k : m, b, tb == 'N' ? n : k, C{}, c.data(), n);
+  return c;
+}
+
+int fail(const std::exception& e) {
+  g_err = e.what();
+  return 1;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Occupied-orbital recovery (NonRelFockMatrixRi.cpp:51-64, X2C_FockMatrixRi.cpp:65-95,
+// UkbFockMatrixRi.cpp:192-209): eigendecompose X^H (S P S) X (scaled), keep eigenvalues > 0.5.
+// ------------------------------------------------------------------------------------------------
+std::vector<double> recoverRealOcc(std::size_t n, std::size_t nx, const double* s, const double* p, const double* x,
+                                   std::size_t& n_occ) {
+  std::vector<double> sps = dmul(n, n, n, dmul(n, n, n, s, p).data(), s);
+  for (double& v : sps) v *= 0.5;
+  const std::vector<double> peff = dmul(nx, n, nx, x, dmul(n, n, nx, sps.data(), x).data(), true, false);
+  std::vector<double> w, v;
+  eigSym(nx, peff.data(), w, v);
+  std::vector<std::size_t> idx;
+  const double thr = planted("threshold") ? 0.25 : 0.5;
+  for (std::size_t i = 0; i < nx; ++i)
+    if (w[i] > thr) idx.push_back(i);
+  n_occ = idx.size();
+  std::vector<double> y(nx * n_occ);
+  for (std::size_t i = 0; i < n_occ; ++i)
+    for (std::size_t a = 0; a < nx; ++a) y[a * n_occ + i] = v[a * nx + idx[i]];
+  return dmul(n, nx, n_occ, x, y.data());  // n x n_occ
+}
+
+std::vector<C> recoverComplexOcc(std::size_t n, std::size_t nx, const C* s, const C* p, const C* x, std::vector<double>& occ) {
+  const std::vector<C> sps = zmul(n, n, n, zmul(n, n, n, s, p).data(), s);
+  const std::vector<C> peff = zmul(nx, n, nx, x, zmul(n, n, nx, sps.data(), x).data(), 'C', 'N');
+  std::vector<double> w;
+  std::vector<C> v;
+  eigHerm(nx, peff.data(), w, v);
+  std::vector<std::size_t> idx;
+  for (std::size_t i = 0; i < nx; ++i)
+    if (w[i] > 0.5) idx.push_back(i);
+  const std::size_t n_occ = idx.size();
+  occ.resize(n_occ);
+  std::vector<C> y(nx * n_occ);
+  for (std::size_t i = 0; i < n_occ; ++i) {
+    occ[i] = w[idx[i]];
+    for (std::size_t a = 0; a < nx; ++a) y[a * n_occ + i] = v[a * nx + idx[i]];
+  }
+  return zmul(n, nx, n_occ, x, y.data());
+}
+
+// Coulomb over one real 3-index tensor: gamma = B vec(P) (added into gamma when accumulate).
+void coulombGamma(std::size_t naux, std::size_t n, const double* b, const double* pflat, double* gamma, bool accumulate) {
+  std::vector<double> g(naux, 0.0);
+  gemv(false, naux, n * n, b, pflat, g.data());
+  for (std::size_t i = 0; i < naux; ++i) gamma[i] = accumulate ? gamma[i] + g[i] : g[i];
+}
+
+// Real exchange K = sum_P (c_t B_P)^T (c_t B_P), c_t = n_occ x n (NonRelFockMatrixRi.cpp:74-94).
+std::vector<double> exchangeReal(std::size_t n, std::size_t naux, const double* b, const double* c_t, std::size_t n_occ) {
+  const int n_parts = omp_get_max_threads();
+  std::vector<std::vector<double>> part(n_parts, std::vector<double>(n * n, 0.0));
+#pragma omp parallel for schedule(static)
+  for (std::size_t p = 0; p < naux; ++p) {
+    if (n_occ == 0) continue;
+    const int tid = omp_get_thread_num();
+    std::vector<double> tmp(n_occ * n, 0.0);
+    gemm(false, false, n_occ, n, n, 1.0, c_t, n, b + p * n * n, n, 0.0, tmp.data(), n);
+    gemm(true, false, n, n, n_occ, 1.0, tmp.data(), n, tmp.data(), n, 1.0, part[tid].data(), n);
+  }
+  std::vector<double> k(n * n, 0.0);
+  for (int t = 0; t < n_parts; ++t)
+    for (std::size_t i = 0; i < n * n; ++i) k[i] += part[t][i];
+  return k;
+}
+
+// Complex multi-flavor exchange (X2C_FockMatrixRi.cpp:97-146, UkbFockMatrixRi.cpp:84-160).
+// Flavor f uses tensor btype[f] (n_b[f] x n_b[f] per aux row) and coefficient rows
+// c_occ[offset[f] + a, i]; returns K blocks for all (f,u) with K_uf = K_fu^H.
+std::vector<std::vector<C>> exchangeFlavors(std::size_t naux, const std::vector<const double*>& tensors,
+                                            const std::vector<std::size_t>& fsize, const std::vector<std::size_t>& offset,
+                                            const std::vector<int>& ftype, const C* c_occ, std::size_t ld_c,
+                                            std::size_t n_occ, const std::vector<double>& occ) {
+  const std::size_t nf = fsize.size();
+  std::vector<std::vector<double>> ct_re(nf), ct_im(nf);
+  for (std::size_t f = 0; f < nf; ++f) {
+    ct_re[f].assign(n_occ * fsize[f], 0.0);
+    ct_im[f].assign(n_occ * fsize[f], 0.0);
+    for (std::size_t i = 0; i < n_occ; ++i) {
+      const double so = std::sqrt(occ[i]);
+      for (std::size_t a = 0; a < fsize[f]; ++a) {
+        const C coeff = c_occ[(offset[f] + a) * ld_c + i];
+        ct_re[f][i * fsize[f] + a] = coeff.real() * so;
+        ct_im[f][i * fsize[f] + a] = (planted("conj") ? coeff.imag() : -coeff.imag()) * so;
+      }
+    }
+  }
+  std::vector<std::pair<std::size_t, std::size_t>> pairs;
+  for (std::size_t f = 0; f < nf; ++f) pairs.emplace_back(f, f);
+  for (std::size_t f = 0; f < nf; ++f)
+    for (std::size_t u = f + 1; u < nf; ++u) pairs.emplace_back(f, u);
+  const int n_parts = omp_get_max_threads();
+  std::vector<std::vector<std::vector<C>>> part(n_parts, std::vector<std::vector<C>>(nf * nf));
+  for (int t = 0; t < n_parts; ++t)
+    for (auto [f, u] : pairs) part[t][f * nf + u].assign(fsize[f] * fsize[u], C{});
+#pragma omp parallel for schedule(static)
+  for (std::size_t p = 0; p < naux; ++p) {
+    const int tid = omp_get_thread_num();
+    std::vector<std::vector<C>> tmp(nf);
+    for (std::size_t f = 0; f < nf; ++f) {
+      const std::size_t fs = fsize[f];
+      std::vector<double> tr(n_occ * fs, 0.0), ti(n_occ * fs, 0.0);
+      const double* slice = tensors[ftype[f]] + p * fs * fs;
+      if (n_occ > 0) {
+        gemm(false, false, n_occ, fs, fs, 1.0, ct_re[f].data(), fs, slice, fs, 0.0, tr.data(), fs);
+        gemm(false, false, n_occ, fs, fs, 1.0, ct_im[f].data(), fs, slice, fs, 0.0, ti.data(), fs);
+      }
+      tmp[f].resize(n_occ * fs);
+      for (std::size_t i = 0; i < n_occ * fs; ++i) tmp[f][i] = C(tr[i], ti[i]);
+    }
+    for (auto [f, u] : pairs) {
+      if (n_occ == 0) continue;
+      zgemm('C', 'N', fsize[f], fsize[u], n_occ, tmp[f].data(), fsize[f], tmp[u].data(), fsize[u], C(1.0, 0.0),
+            part[tid][f * nf + u].data(), fsize[u]);
+    }
+  }
+  std::vector<std::vector<C>> km(nf * nf);
+  for (auto [f, u] : pairs) {
+    km[f * nf + u].assign(fsize[f] * fsize[u], C{});
+    for (int t = 0; t < n_parts; ++t)
+      for (std::size_t i = 0; i < km[f * nf + u].size(); ++i) km[f * nf + u][i] += part[t][f * nf + u][i];
+    if (f != u) {
+      km[u * nf + f].assign(fsize[u] * fsize[f], C{});
+      for (std::size_t a = 0; a < fsize[f]; ++a)
+        for (std::size_t b = 0; b < fsize[u]; ++b) km[u * nf + f][b * fsize[f] + a] = std::conj(km[f * nf + u][a * fsize[u] + b]);
+    }
+  }
+  return km;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_err.c_str(); }
+int oracle_has_mkl() { return blas().mkl ? 1 : 0; }
+int oracle_max_threads() { return omp_get_max_threads(); }
+
+// nonRelFockMatrix(h, RiNonRelEriSource, P): src/NON_REL/NonRelFockMatrixRi.cpp:27-100.
+// Optional outputs j_out/k_out (may be null) hold J and the raw K (F = h + J - K/2).
+int oracle_nonrel_fock_ri(std::size_t nl, std::size_t naux, const double* eri3, const double* x_large, std::size_t nx,
+                          const double* overlap, const double* h_core, const double* density, double* fock,
+                          double* j_out, double* k_out) {
+  try {
+    std::vector<double> gamma(naux);
+    coulombGamma(naux, nl, eri3, density, gamma.data(), false);  // :44-45
+    std::vector<double> j(nl * nl, 0.0);
+    gemv(true, naux, nl * nl, eri3, gamma.data(), j.data());  // :47-48
+    std::size_t n_occ = 0;
+    const std::vector<double> c_occ = recoverRealOcc(nl, nx, overlap, density, x_large, n_occ);  // :52-64
+    std::vector<double> c_t(n_occ * nl);
+    for (std::size_t i = 0; i < n_occ; ++i)
+      for (std::size_t q = 0; q < nl; ++q)
+        c_t[i * nl + q] = (planted("sqrt2") ? 1.0 : std::sqrt(2.0)) * c_occ[q * n_occ + i];  // :67-69
+    const std::vector<double> k = exchangeReal(nl, naux, eri3, c_t.data(), n_occ);
+    const double kf = planted("half") ? 1.0 : 0.5;
+    for (std::size_t i = 0; i < nl * nl; ++i) fock[i] = h_core[i] + j[i] - kf * k[i];  // :96-99
+    if (j_out) std::copy(j.begin(), j.end(), j_out);
+    if (k_out) std::copy(k.begin(), k.end(), k_out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// Factor-form real J/K with no recovery: J = sum_P tr(B_P D) B_P; K = sum_P (L^T B_P)^T (R^T B_P),
+// i.e. K = sum_P B_P^T (L R^T) B_P with L, R n x r (the closed form used for incremental and
+// low-rank density builds; with L = R = sqrt(2) C_occ it is the exchange of
+// NonRelFockMatrixRi.cpp:74-94).
+int oracle_ri_jk_factor(std::size_t n, std::size_t naux, const double* eri3, const double* d, const double* lfac,
+                        const double* rfac, std::size_t r, double* j_out, double* k_out) {
+  try {
+    std::vector<double> gamma(naux);
+    coulombGamma(naux, n, eri3, d, gamma.data(), false);
+    gemv(true, naux, n * n, eri3, gamma.data(), j_out);
+    const int n_parts = omp_get_max_threads();
+    std::vector<std::vector<double>> part(n_parts, std::vector<double>(n * n, 0.0));
+    std::vector<double> lt(r * n), rt(r * n);
+    for (std::size_t i = 0; i < r; ++i)
+      for (std::size_t q = 0; q < n; ++q) lt[i * n + q] = lfac[q * r + i], rt[i * n + q] = rfac[q * r + i];
+#pragma omp parallel for schedule(static)
+    for (std::size_t p = 0; p < naux; ++p) {
+      if (r == 0) continue;
+      const int tid = omp_get_thread_num();
+      std::vector<double> tl(r * n), tr(r * n);
+      gemm(false, false, r, n, n, 1.0, lt.data(), n, eri3 + p * n * n, n, 0.0, tl.data(), n);
+      gemm(false, false, r, n, n, 1.0, rt.data(), n, eri3 + p * n * n, n, 0.0, tr.data(), n);
+      gemm(true, false, n, n, r, 1.0, tl.data(), n, tr.data(), n, 1.0, part[tid].data(), n);
+    }
+    std::fill(k_out, k_out + n * n, 0.0);
+    for (int t = 0; t < n_parts; ++t)
+      for (std::size_t i = 0; i < n * n; ++i) k_out[i] += part[t][i];
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// x2cFockMatrix(h, RiX2cEriSource, P): src/X2C_DHF/X2C_FockMatrixRi.cpp:35-162. Complex arrays are
+// interleaved (re, im) doubles, row-major.
+int oracle_x2c_fock_ri(std::size_t nl, std::size_t naux, const double* eri3, const double* x_large_, std::size_t nx,
+                       const double* overlap_, const double* h_, const double* density_, double* fock_) {
+  try {
+    const C* x_large = reinterpret_cast<const C*>(x_large_);
+    const C* overlap = reinterpret_cast<const C*>(overlap_);
+    const C* h = reinterpret_cast<const C*>(h_);
+    const C* dens = reinterpret_cast<const C*>(density_);
+    C* fock = reinterpret_cast<C*>(fock_);
+    const std::size_t n = 2 * nl;
+    std::vector<double> pflat(nl * nl);
+    for (std::size_t mu = 0; mu < nl; ++mu)
+      for (std::size_t nu = 0; nu < nl; ++nu)
+        pflat[mu * nl + nu] = (dens[mu * n + nu] + dens[(nl + mu) * n + nl + nu]).real();  // :50-55
+    std::vector<double> gamma(naux), j(nl * nl);
+    coulombGamma(naux, nl, eri3, pflat.data(), gamma.data(), false);
+    gemv(true, naux, nl * nl, eri3, gamma.data(), j.data());
+    std::vector<double> occ;
+    const std::vector<C> c_occ = recoverComplexOcc(n, nx, overlap, dens, x_large, occ);  // :65-95
+    const std::size_t n_occ = occ.size();
+    const auto km = exchangeFlavors(naux, {eri3}, {nl, nl}, {0, nl}, {0, 0}, c_occ.data(), n_occ, n_occ, occ);
+    for (std::size_t i = 0; i < n * n; ++i) fock[i] = h[i];
+    for (std::size_t a = 0; a < nl; ++a)
+      for (std::size_t b = 0; b < nl; ++b) {
+        fock[a * n + b] += j[a * nl + b];
+        fock[(nl + a) * n + nl + b] += j[a * nl + b];
+      }
+    for (std::size_t f = 0; f < 2; ++f)
+      for (std::size_t u = 0; u < 2; ++u)
+        for (std::size_t a = 0; a < nl; ++a)
+          for (std::size_t b = 0; b < nl; ++b) fock[(f * nl + a) * n + u * nl + b] -= km[f * 2 + u][a * nl + b];
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// riFockTwoElectronDirect: src/UKB/UkbFockMatrixRi.cpp:35-183 (G = J - K in the UKB basis).
+int oracle_ri_fock_two_electron_direct(std::size_t nl, std::size_t ns, std::size_t naux, const double* eri3_ll,
+                                       const double* eri3_ss, const double* density_ukb_, const double* c_occ_ukb_,
+                                       std::size_t n_occ, const double* occupations, double* g_) {
+  try {
+    const C* dens = reinterpret_cast<const C*>(density_ukb_);
+    const C* c_occ = reinterpret_cast<const C*>(c_occ_ukb_);
+    C* g = reinterpret_cast<C*>(g_);
+    const std::size_t n = 2 * nl + 2 * ns;
+    const std::vector<std::size_t> offset = {0, nl, 2 * nl, 2 * nl + ns}, fsize = {nl, nl, ns, ns};
+    std::vector<double> pl(nl * nl), ps(ns * ns);
+    for (std::size_t a = 0; a < nl; ++a)
+      for (std::size_t b = 0; b < nl; ++b) pl[a * nl + b] = (dens[a * n + b] + dens[(nl + a) * n + nl + b]).real();
+    for (std::size_t a = 0; a < ns; ++a)
+      for (std::size_t b = 0; b < ns; ++b)
+        ps[a * ns + b] = (dens[(2 * nl + a) * n + 2 * nl + b] + dens[(2 * nl + ns + a) * n + 2 * nl + ns + b]).real();
+    std::vector<double> gamma(naux, 0.0), jl(nl * nl), js(ns * ns);
+    coulombGamma(naux, nl, eri3_ll, pl.data(), gamma.data(), false);  // :70-74 shared X_P
+    coulombGamma(naux, ns, eri3_ss, ps.data(), gamma.data(), true);
+    gemv(true, naux, nl * nl, eri3_ll, gamma.data(), jl.data());
+    gemv(true, naux, ns * ns, eri3_ss, gamma.data(), js.data());
+    const std::vector<double> occ(occupations, occupations + n_occ);
+    const auto km = exchangeFlavors(naux, {eri3_ll, eri3_ss}, fsize, offset, {0, 0, 1, 1}, c_occ, n_occ, n_occ, occ);
+    for (std::size_t i = 0; i < n * n; ++i) g[i] = C{};
+    for (std::size_t a = 0; a < nl; ++a)
+      for (std::size_t b = 0; b < nl; ++b) g[a * n + b] += jl[a * nl + b], g[(nl + a) * n + nl + b] += jl[a * nl + b];
+    for (std::size_t a = 0; a < ns; ++a)
+      for (std::size_t b = 0; b < ns; ++b)
+        g[(2 * nl + a) * n + 2 * nl + b] += js[a * ns + b], g[(2 * nl + ns + a) * n + 2 * nl + ns + b] += js[a * ns + b];
+    for (std::size_t f = 0; f < 4; ++f)
+      for (std::size_t u = 0; u < 4; ++u)
+        for (std::size_t a = 0; a < fsize[f]; ++a)
+          for (std::size_t b = 0; b < fsize[u]; ++b) g[(offset[f] + a) * n + offset[u] + b] -= km[f * 4 + u][a * fsize[u] + b];
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// rkbFockMatrix(h, RiDirectEriSource, P): src/UKB/UkbFockMatrixRi.cpp:185-216.
+int oracle_rkb_fock_ri(std::size_t nl, std::size_t ns, std::size_t naux, const double* eri3_ll, const double* eri3_ss,
+                       const double* v_total_, std::size_t n_rkb, const double* s_full_, const double* x_full_,
+                       std::size_t nx, const double* h_rkb_, const double* density_, double* fock_) {
+  try {
+    const C* v = reinterpret_cast<const C*>(v_total_);
+    const C* s = reinterpret_cast<const C*>(s_full_);
+    const C* x = reinterpret_cast<const C*>(x_full_);
+    const C* h = reinterpret_cast<const C*>(h_rkb_);
+    const C* dens = reinterpret_cast<const C*>(density_);
+    C* fock = reinterpret_cast<C*>(fock_);
+    const std::size_t n_ukb = 2 * nl + 2 * ns;
+    std::vector<double> occ;
+    const std::vector<C> c_rkb = recoverComplexOcc(n_rkb, nx, s, dens, x, occ);
+    const std::size_t n_occ = occ.size();
+    const std::vector<C> c_ukb = zmul(n_ukb, n_rkb, n_occ, v, c_rkb.data());
+    const std::vector<C> p_ukb = zmul(n_ukb, n_rkb, n_ukb, v, zmul(n_rkb, n_rkb, n_ukb, dens, v, 'N', 'C').data());
+    std::vector<C> g(n_ukb * n_ukb);
+    int rc = oracle_ri_fock_two_electron_direct(nl, ns, naux, eri3_ll, eri3_ss, reinterpret_cast<const double*>(p_ukb.data()),
+                                                reinterpret_cast<const double*>(c_ukb.data()), n_occ, occ.data(),
+                                                reinterpret_cast<double*>(g.data()));
+    if (rc) return rc;
+    const std::vector<C> vgv = zmul(n_rkb, n_ukb, n_rkb, v, zmul(n_ukb, n_ukb, n_rkb, g.data(), v).data(), 'C', 'N');
+    for (std::size_t i = 0; i < n_rkb * n_rkb; ++i) fock[i] = h[i] + vgv[i];
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// buildRiMoEri: src/UKB/RiMoEri.cpp:20-86. out = (naux, n_total^2) complex.
+int oracle_build_ri_mo_eri(std::size_t nl, std::size_t ns, std::size_t naux, const double* eri3_ll, const double* eri3_ss,
+                           const double* c_ukb_, std::size_t n_total, double* out_) {
+  try {
+    const C* c_ukb = reinterpret_cast<const C*>(c_ukb_);
+    C* out = reinterpret_cast<C*>(out_);
+    const std::size_t offset[4] = {0, nl, 2 * nl, 2 * nl + ns}, fsize[4] = {nl, nl, ns, ns};
+    std::fill(out, out + naux * n_total * n_total, C{});
+    for (int f = 0; f < 4; ++f) {
+      const std::size_t np = fsize[f];
+      if (np == 0 || n_total == 0) continue;
+      const double* tens = f < 2 ? eri3_ll : eri3_ss;
+      std::vector<double> cdr(n_total * np), cdi(n_total * np);
+      std::vector<C> cblock(np * n_total);
+      for (std::size_t a = 0; a < np; ++a)
+        for (std::size_t p = 0; p < n_total; ++p) {
+          const C v = c_ukb[(offset[f] + a) * n_total + p];
+          cblock[a * n_total + p] = v;
+          cdr[p * np + a] = v.real();
+          cdi[p * np + a] = planted("mo_conj") ? v.imag() : -v.imag();
+        }
+#pragma omp parallel for schedule(dynamic)
+      for (std::size_t P = 0; P < naux; ++P) {
+        const double* slice = tens + P * np * np;
+        std::vector<double> tr(n_total * np), ti(n_total * np);
+        gemm(false, false, n_total, np, np, 1.0, cdr.data(), np, slice, np, 0.0, tr.data(), np);
+        gemm(false, false, n_total, np, np, 1.0, cdi.data(), np, slice, np, 0.0, ti.data(), np);
+        std::vector<C> t(n_total * np);
+        for (std::size_t i = 0; i < t.size(); ++i) t[i] = C(tr[i], ti[i]);
+        std::vector<C> blk(n_total * n_total);
+        zgemm('N', 'N', n_total, n_total, np, t.data(), np, cblock.data(), n_total, C{}, blk.data(), n_total);
+        C* dst = out + P * n_total * n_total;
+        for (std::size_t i = 0; i < blk.size(); ++i) dst[i] += blk[i];
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// RiMoEri::rotated: src/UKB/RiMoEri.cpp:88-110. B'_P = U^H B_P U.
+int oracle_ri_mo_eri_rotated(std::size_t n_total, std::size_t naux, const double* b_, const double* u_, double* out_) {
+  try {
+    const C* b = reinterpret_cast<const C*>(b_);
+    const C* u = reinterpret_cast<const C*>(u_);
+    C* out = reinterpret_cast<C*>(out_);
+    const std::size_t n = n_total;
+#pragma omp parallel for schedule(dynamic)
+    for (std::size_t P = 0; P < naux; ++P) {
+      std::vector<C> t(n * n);
+      zgemm(planted("rot_dagger") ? 'T' : 'C', 'N', n, n, n, u, n, b + P * n * n, n, C{}, t.data(), n);
+      zgemm('N', 'N', n, n, n, t.data(), n, u, n, C{}, out + P * n * n, n);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// buildRiNonRelSpinMoEri: src/NON_REL/NonRelSpinRiMoEri.cpp:13-43. B^MO_P = C^T B_P C (real).
+int oracle_build_ri_nonrel_spin_mo_eri(std::size_t n_spatial, std::size_t naux, const double* eri3, const double* c,
+                                       double* out) {
+  try {
+    const std::size_t n = n_spatial;
+    std::vector<double> ct(n * n);
+    for (std::size_t i = 0; i < n; ++i)
+      for (std::size_t j = 0; j < n; ++j) ct[i * n + j] = c[j * n + i];
+#pragma omp parallel for schedule(dynamic)
+    for (std::size_t P = 0; P < naux; ++P) {
+      std::vector<double> t(n * n);
+      gemm(false, false, n, n, n, 1.0, ct.data(), n, eri3 + P * n * n, n, 0.0, t.data(), n);
+      gemm(false, false, n, n, n, 1.0, t.data(), n, c, n, 0.0, out + P * n * n, n);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// RiNonRelSpinMoEri::rotated: src/NON_REL/NonRelSpinRiMoEri.cpp:45-95. u is 2n x 2n, must be
+// blockdiag(u_s, u_s) to 1e-3; the two spin blocks are averaged. Returns 2 when u is not
+// spin-restricted (the reference throws).
+int oracle_ri_nonrel_spin_mo_eri_rotated(std::size_t n_spatial, std::size_t naux, const double* b, const double* u,
+                                         double* out) {
+  try {
+    const std::size_t n = n_spatial, n2 = 2 * n;
+    double max_off = 0.0, max_mis = 0.0;
+    for (std::size_t i = 0; i < n; ++i)
+      for (std::size_t j = 0; j < n; ++j) {
+        max_off = std::max({max_off, std::abs(u[i * n2 + n + j]), std::abs(u[(n + i) * n2 + j])});
+        max_mis = std::max(max_mis, std::abs(u[(n + i) * n2 + n + j] - u[i * n2 + j]));
+      }
+    if (max_off > 1e-3 || max_mis > 1e-3) {
+      g_err = "RiNonRelSpinMoEri::rotated: rotation is not spin-restricted";
+      return 2;
+    }
+    std::vector<double> us(n * n);
+    for (std::size_t i = 0; i < n; ++i)
+      for (std::size_t j = 0; j < n; ++j) us[i * n + j] = 0.5 * (u[i * n2 + j] + u[(n + i) * n2 + n + j]);
+    return oracle_build_ri_nonrel_spin_mo_eri(n, naux, b, us.data(), out);
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// nonRelFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:108-149. vectors = (nchol, n*n).
+int oracle_nonrel_fock_cholesky(std::size_t n, std::size_t nchol, const double* vecs, const double* h, const double* p,
+                                double* fock) {
+  try {
+    const int nt = omp_get_max_threads();
+    std::vector<std::vector<double>> jp(nt, std::vector<double>(n * n, 0.0)), kp(nt, std::vector<double>(n * n, 0.0));
+#pragma omp parallel
+    {
+      const int tid = omp_get_thread_num();
+      std::vector<double> tmp(n * n);
+#pragma omp for schedule(static)
+      for (std::size_t l = 0; l < nchol; ++l) {
+        const double* b = vecs + l * n * n;
+        double t = 0.0;
+        for (std::size_t q = 0; q < n; ++q)
+          for (std::size_t s = 0; s < n; ++s) t += b[q * n + s] * p[s * n + q];
+        for (std::size_t i = 0; i < n * n; ++i) jp[tid][i] += t * b[i];
+        gemm(false, planted("chol_transpose"), n, n, n, 1.0, p, n, b, n, 0.0, tmp.data(), n);
+        gemm(false, false, n, n, n, 1.0, b, n, tmp.data(), n, 1.0, kp[tid].data(), n);
+      }
+    }
+    for (std::size_t i = 0; i < n * n; ++i) {
+      double j = 0.0, k = 0.0;
+      for (int t = 0; t < nt; ++t) j += jp[t][i], k += kp[t][i];
+      fock[i] = h[i] + j - 0.5 * k;
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// x2cFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:151-217.
+int oracle_x2c_fock_cholesky(std::size_t n, std::size_t nchol, const double* vecs, const double* h_, const double* p_,
+                             double* fock_) {
+  try {
+    const C* h = reinterpret_cast<const C*>(h_);
+    const C* p = reinterpret_cast<const C*>(p_);
+    C* fock = reinterpret_cast<C*>(fock_);
+    const std::size_t n2 = 2 * n;
+    std::vector<C> pb[2][2];
+    for (int s = 0; s < 2; ++s)
+      for (int t = 0; t < 2; ++t) {
+        pb[s][t].resize(n * n);
+        for (std::size_t a = 0; a < n; ++a)
+          for (std::size_t b = 0; b < n; ++b) pb[s][t][a * n + b] = p[(s * n + a) * n2 + t * n + b];
+      }
+    const int nt = omp_get_max_threads();
+    std::vector<std::vector<C>> jp(nt, std::vector<C>(n * n)), kp(nt * 4, std::vector<C>(n * n));
+#pragma omp parallel
+    {
+      const int tid = omp_get_thread_num();
+      std::vector<C> bc(n * n), tmp(n * n);
+#pragma omp for schedule(static)
+      for (std::size_t l = 0; l < nchol; ++l) {
+        const double* b = vecs + l * n * n;
+        for (std::size_t i = 0; i < n * n; ++i) bc[i] = C(b[i], 0.0);
+        C t{};
+        for (int s = 0; s < 2; ++s)
+          for (std::size_t q = 0; q < n; ++q)
+            for (std::size_t r = 0; r < n; ++r) t += b[q * n + r] * pb[s][s][r * n + q];
+        for (std::size_t i = 0; i < n * n; ++i) jp[tid][i] += t * b[i];
+        for (int s = 0; s < 2; ++s)
+          for (int u = 0; u < 2; ++u) {
+            zgemm('N', 'N', n, n, n, pb[s][u].data(), n, bc.data(), n, C{}, tmp.data(), n);
+            zgemm('N', 'N', n, n, n, bc.data(), n, tmp.data(), n, C(1.0, 0.0), kp[tid * 4 + s * 2 + u].data(), n);
+          }
+      }
+    }
+    std::vector<C> jm(n * n), km[4];
+    for (int q = 0; q < 4; ++q) km[q].assign(n * n, C{});
+    for (int t = 0; t < nt; ++t)
+      for (std::size_t i = 0; i < n * n; ++i) {
+        jm[i] += jp[t][i];
+        for (int q = 0; q < 4; ++q) km[q][i] += kp[t * 4 + q][i];
+      }
+    for (std::size_t i = 0; i < n2 * n2; ++i) fock[i] = h[i];
+    for (std::size_t a = 0; a < n; ++a)
+      for (std::size_t b = 0; b < n; ++b)
+        for (int s = 0; s < 2; ++s) {
+          fock[(s * n + a) * n2 + s * n + b] += jm[a * n + b];
+          for (int u = 0; u < 2; ++u) fock[(s * n + a) * n2 + u * n + b] -= km[s * 2 + u][a * n + b];
+        }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// aoCholeskyToMoSpinOrbital: src/Utils/AoCholesky.cpp:219-243. Returns the spatial C^T B_L C
+// (nchol, nmo*nmo); the reference's spin-orbital vector is blockdiag(W, W) of it.
+int oracle_cholesky_to_mo_spatial(std::size_t n, std::size_t nchol, const double* vecs, const double* c, std::size_t nmo,
+                                  double* out) {
+  try {
+#pragma omp parallel
+    {
+      std::vector<double> tmp(n * nmo);
+#pragma omp for schedule(dynamic)
+      for (std::size_t l = 0; l < nchol; ++l) {
+        gemm(false, false, n, nmo, n, 1.0, vecs + l * n * n, n, c, nmo, 0.0, tmp.data(), nmo);
+        gemm(true, false, nmo, nmo, n, 1.0, c, nmo, tmp.data(), nmo, 0.0, out + l * nmo * nmo, nmo);
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// aoCholeskyToMoSpinor: src/Utils/AoCholesky.cpp:270-295. c is 2n x nmo complex; out W_L = (sum_s
+// C_s^H B_L C_s)^T, (nchol, nmo*nmo) complex.
+int oracle_cholesky_to_mo_spinor(std::size_t n, std::size_t nchol, const double* vecs, const double* c_, std::size_t nmo,
+                                 double* out_) {
+  try {
+    const C* c = reinterpret_cast<const C*>(c_);
+    C* out = reinterpret_cast<C*>(out_);
+#pragma omp parallel
+    {
+      std::vector<C> bc(n * n), tmp(n * nmo), bp(nmo * nmo);
+#pragma omp for schedule(dynamic)
+      for (std::size_t l = 0; l < nchol; ++l) {
+        for (std::size_t i = 0; i < n * n; ++i) bc[i] = C(vecs[l * n * n + i], 0.0);
+        for (int s = 0; s < 2; ++s) {
+          const C* cs = c + (std::size_t)s * n * nmo;
+          zgemm('N', 'N', n, nmo, n, bc.data(), n, cs, nmo, C{}, tmp.data(), nmo);
+          zgemm('C', 'N', nmo, nmo, n, cs, nmo, tmp.data(), nmo, s == 0 ? C{} : C(1.0, 0.0), bp.data(), nmo);
+        }
+        for (std::size_t x = 0; x < nmo; ++x)
+          for (std::size_t y = 0; y < nmo; ++y) out[l * nmo * nmo + x * nmo + y] = bp[y * nmo + x];
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// Hermitian eigensolve exposed for tests (LinearAlgebra.cpp:125-150 semantics).
+int oracle_eig_hermitian(std::size_t n, const double* a_, double* w, double* v_) {
+  try {
+    std::vector<double> ww;
+    std::vector<C> vv;
+    eigHerm(n, reinterpret_cast<const C*>(a_), ww, vv);
+    std::copy(ww.begin(), ww.end(), w);
+    std::copy(vv.begin(), vv.end(), reinterpret_cast<C*>(v_));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+"""rerdmft_amd: MI355X-native RI / Cholesky J/K Fock builds and RI AO->MO transforms.
+
+Python front-end over the C ABI of librerdmft_gpu.so (include/rerdmft_gpu.h). The function names
+mirror the reference's C++ overload set (nonRelFockMatrix, x2cFockMatrix, rkbFockMatrix,
+riFockTwoElectronDirect, buildRiMoEri, ...), with the same argument meaning and the same shape
+errors (raised as RuntimeError). Arrays may be numpy (host) or torch device tensors; results are
+returned as numpy arrays unless an `out=` device tensor is given.
+
+There is no CPU fallback: importing succeeds without a GPU, but creating a Context requires the
+built library and a visible device, and fails loudly otherwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librerdmft_gpu.so")
+
+FP64, FP32, BF16 = 0, 1, 2
+COPY_FP64, COPY_FP32, COPY_BF16 = 1, 2, 4
+PRECISION_NAMES = {FP64: "f64", FP32: "f32", BF16: "bf16"}
+
+# Exported C symbols declared in include/rerdmft_gpu.h (checked by the CPU test suite).
+EXPORTED = [
+    "rr_create", "rr_destroy", "rr_last_error", "rr_set_tile_config", "rr_timing",
+    "rr_tensor_upload", "rr_tensor_generate", "rr_tensor_info", "rr_tensor_release",
+    "rr_nonrel_fock_ri", "rr_ri_jk_factor", "rr_ri_jk_delta", "rr_x2c_fock_ri",
+    "rr_ri_fock_two_electron_direct", "rr_rkb_fock_ri", "rr_build_ri_mo_eri", "rr_ri_mo_eri_rotated",
+    "rr_ri_mo_eri_positive_block", "rr_build_ri_nonrel_spin_mo_eri", "rr_ri_nonrel_spin_mo_eri_rotated",
+    "rr_ri_nonrel_spin_embedded_b", "rr_nonrel_fock_cholesky", "rr_x2c_fock_cholesky",
+    "rr_cholesky_to_mo_spatial", "rr_cholesky_to_mo_spinor",
+]
+
+_lib = None
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    """Load librerdmft_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rerdmft_amd.build`")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.rr_last_error.restype = ctypes.c_char_p
+        _lib.rr_last_error.argtypes = [ctypes.c_void_p]
+        _lib.rr_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        _lib.rr_destroy.argtypes = [ctypes.c_void_p]
+    return _lib
+
+
+_sz = ctypes.c_size_t
+
+
+class _Arr:
+    """Pointer view of a numpy array or torch tensor (keeps a contiguous copy alive)."""
+
+    def __init__(self, x, dtype):
+        self.torch = False
+        if x is None:
+            self.ptr = ctypes.c_void_p(0)
+            self.obj = None
+            return
+        if type(x).__module__.startswith("torch"):
+            import torch
+            want = {np.float64: torch.float64, np.complex128: torch.complex128, np.float32: torch.float32}[dtype]
+            t = x.contiguous()
+            if t.dtype != want:
+                t = t.to(want)
+            self.obj = t
+            self.ptr = ctypes.c_void_p(t.data_ptr())
+            self.torch = True
+        else:
+            a = np.ascontiguousarray(x, dtype=dtype)
+            self.obj = a
+            self.ptr = ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(x):
+    return _Arr(x, np.float64)
+
+
+def _c128(x):
+    return _Arr(x, np.complex128)
+
+
+def _out(out, shape, dtype):
+    if out is not None:
+        return out, _Arr(out, dtype).ptr
+    a = np.empty(shape, dtype=dtype)
+    return a, ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One device context (stream, solver handle, resident 3-index tensors in numbered slots)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().rr_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise GpuError(rc, f"rr_create failed on device {device} (code {rc})")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _call(self, name, *args):
+        rc = getattr(lib(), name)(self._h, *args)
+        if rc != 0:
+            raise GpuError(rc, lib().rr_last_error(self._h).decode())
+
+    # ---- configuration / timing ----------------------------------------------------------------
+    def set_tile_config(self, cfg):
+        self._call("rr_set_tile_config", ctypes.c_int(cfg))
+
+    def timing(self, enable=True, reset=False):
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_longlong * 8)()
+        self._call("rr_timing", ctypes.c_int(int(enable)), ctypes.c_int(int(reset)), ms, cnt, ctypes.c_int(8))
+        names = ["gamma", "coulomb", "exchange_half", "exchange_rank_k", "assembly", "recovery"]
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
+
+    # ---- tensors -------------------------------------------------------------------------------
+    def upload(self, slot, eri3, copies=COPY_FP64):
+        a = _f64(eri3)
+        shape = tuple(eri3.shape)
+        naux = shape[0]
+        n2 = int(np.prod(shape[1:]))
+        n = int(round(np.sqrt(n2)))
+        if n * n != n2:
+            raise ValueError("eri3 must be (n_aux, n*n) or (n_aux, n, n)")
+        self._call("rr_tensor_upload", ctypes.c_int(slot), a.ptr, _sz(naux), _sz(n), ctypes.c_uint(copies))
+
+    def generate(self, slot, naux, n, sigma, decay, seed, copies=COPY_FP64):
+        self._call("rr_tensor_generate", ctypes.c_int(slot), _sz(naux), _sz(n), ctypes.c_double(sigma),
+                   ctypes.c_double(decay), ctypes.c_uint64(seed), ctypes.c_uint(copies))
+
+    def tensor_info(self, slot):
+        na, n, cp = _sz(), _sz(), ctypes.c_uint()
+        self._call("rr_tensor_info", ctypes.c_int(slot), ctypes.byref(na), ctypes.byref(n), ctypes.byref(cp))
+        return na.value, n.value, cp.value
+
+    def release(self, slot):
+        self._call("rr_tensor_release", ctypes.c_int(slot))
+
+    # ---- NON_REL -------------------------------------------------------------------------------
+    def nonrel_fock_ri(self, slot, h_core, x_large, overlap, density, precision=FP64, return_jk=False, out=None):
+        """nonRelFockMatrix(h, RiNonRelEriSource, P) -- src/NON_REL/NonRelFockMatrixRi.cpp:27-100."""
+        n = h_core.shape[0]
+        h, x, s, d = _f64(h_core), _f64(x_large), _f64(overlap), _f64(density)
+        f, fp = _out(out, (n, n), np.float64)
+        j = k = None
+        jp = kp = ctypes.c_void_p(0)
+        if return_jk:
+            j, jp = _out(None, (n, n), np.float64)
+            k, kp = _out(None, (n, n), np.float64)
+        self._call("rr_nonrel_fock_ri", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), h.ptr, x.ptr,
+                   _sz(x_large.shape[1]), s.ptr, d.ptr, fp, jp, kp)
+        return (f, j, k) if return_jk else f
+
+    def ri_jk_factor(self, slot, density, lfac, rfac=None, precision=FP64, want_j=True, want_k=True, j_out=None,
+                     k_out=None):
+        """J = sum_P tr(B_P D) B_P, K = sum_P B_P^T (L R^T) B_P."""
+        n = lfac.shape[0] if lfac is not None else density.shape[0]
+        r = lfac.shape[1] if lfac is not None else 0
+        d, l, rr_ = _f64(density), _f64(lfac), _f64(rfac)
+        j, jp = _out(j_out, (n, n), np.float64) if want_j else (None, ctypes.c_void_p(0))
+        k, kp = _out(k_out, (n, n), np.float64) if want_k else (None, ctypes.c_void_p(0))
+        self._call("rr_ri_jk_factor", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), d.ptr, l.ptr, rr_.ptr,
+                   _sz(r), jp, kp)
+        return j, k
+
+    def ri_jk_delta(self, slot, c_new, c_old, precision=FP64, dj_out=None, dk_out=None):
+        """dJ, dK of D_new - D_old with D = 2 C C^T (aligned-difference factorization)."""
+        n, no = c_new.shape
+        a, b = _f64(c_new), _f64(c_old)
+        dj, djp = _out(dj_out, (n, n), np.float64)
+        dk, dkp = _out(dk_out, (n, n), np.float64)
+        self._call("rr_ri_jk_delta", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), _sz(no), a.ptr, b.ptr, djp,
+                   dkp)
+        return dj, dk
+
+    # ---- X2C -----------------------------------------------------------------------------------
+    def x2c_fock_ri(self, slot, h_x2c, x_large, overlap, density, precision=FP64, out=None):
+        """x2cFockMatrix(h, RiX2cEriSource, P) -- src/X2C_DHF/X2C_FockMatrixRi.cpp:35-162."""
+        n2 = h_x2c.shape[0]
+        h, x, s, d = _c128(h_x2c), _c128(x_large), _c128(overlap), _c128(density)
+        f, fp = _out(out, (n2, n2), np.complex128)
+        self._call("rr_x2c_fock_ri", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n2 // 2), h.ptr, x.ptr,
+                   _sz(x_large.shape[1]), s.ptr, d.ptr, fp)
+        return f
+
+    # ---- RKB -----------------------------------------------------------------------------------
+    def ri_fock_two_electron_direct(self, slot_ll, slot_ss, nl, ns, density_ukb, c_occ_ukb, occupations,
+                                    precision=FP64, out=None):
+        """riFockTwoElectronDirect -- src/UKB/UkbFockMatrixRi.cpp:35-183."""
+        n = 2 * nl + 2 * ns
+        d, c = _c128(density_ukb), _c128(c_occ_ukb)
+        occ = _f64(occupations)
+        g, gp = _out(out, (n, n), np.complex128)
+        self._call("rr_ri_fock_two_electron_direct", ctypes.c_int(slot_ll), ctypes.c_int(slot_ss),
+                   ctypes.c_int(precision), _sz(nl), _sz(ns), d.ptr, c.ptr, _sz(len(occupations)), occ.ptr, gp)
+        return g
+
+    def rkb_fock_ri(self, slot_ll, slot_ss, nl, ns, h_rkb, v_total, s_full, x_full, density, precision=FP64,
+                    out=None):
+        """rkbFockMatrix(h, RiDirectEriSource, P) -- src/UKB/UkbFockMatrixRi.cpp:185-216."""
+        n_rkb = h_rkb.shape[0]
+        h, v, s, x, d = map(_c128, (h_rkb, v_total, s_full, x_full, density))
+        f, fp = _out(out, (n_rkb, n_rkb), np.complex128)
+        self._call("rr_rkb_fock_ri", ctypes.c_int(slot_ll), ctypes.c_int(slot_ss), ctypes.c_int(precision), _sz(nl),
+                   _sz(ns), h.ptr, v.ptr, _sz(n_rkb), s.ptr, x.ptr, _sz(x_full.shape[1]), d.ptr, fp)
+        return f
+
+    # ---- MO transforms -------------------------------------------------------------------------
+    def build_ri_mo_eri(self, slot_ll, slot_ss, nl, ns, c_ukb, precision=FP64, out_precision=FP64, out=None):
+        """buildRiMoEri -- src/UKB/RiMoEri.cpp:64-86. Returns (n_aux, n_total, n_total) complex."""
+        naux = self.tensor_info(slot_ll)[0]
+        nt = c_ukb.shape[1]
+        c = _c128(c_ukb)
+        dt = np.complex128 if out_precision == FP64 else np.complex64
+        if out is not None:
+            o, op = out, ctypes.c_void_p(out.data_ptr())
+        else:
+            o = np.empty((naux, nt, nt), dtype=dt)
+            op = ctypes.c_void_p(o.ctypes.data)
+        self._call("rr_build_ri_mo_eri", ctypes.c_int(slot_ll), ctypes.c_int(slot_ss), ctypes.c_int(precision),
+                   ctypes.c_int(out_precision), _sz(nl), _sz(ns), c.ptr, _sz(nt), op)
+        return o
+
+    def ri_mo_eri_rotated(self, b, u, precision=FP64, out=None):
+        """RiMoEri::rotated -- src/UKB/RiMoEri.cpp:88-110."""
+        naux, nt = b.shape[0], u.shape[0]
+        bb, uu = _c128(b), _c128(u)
+        o, op = _out(out, (naux, nt, nt), np.complex128)
+        self._call("rr_ri_mo_eri_rotated", ctypes.c_int(precision), _sz(nt), _sz(naux), bb.ptr, uu.ptr, op)
+        return o
+
+    def ri_mo_eri_positive_block(self, b, off, out=None):
+        """RiMoEri::positiveBlock -- src/UKB/RiMoEri.cpp:112-121."""
+        naux, nt = b.shape[0], b.shape[1]
+        m = nt - off
+        bb = _c128(b)
+        o, op = _out(out, (naux, m, m), np.complex128)
+        self._call("rr_ri_mo_eri_positive_block", _sz(nt), _sz(naux), bb.ptr, _sz(off), op)
+        return o
+
+    def build_ri_nonrel_spin_mo_eri(self, slot, c_spatial, precision=FP64, out=None):
+        """buildRiNonRelSpinMoEri -- src/NON_REL/NonRelSpinRiMoEri.cpp:13-43."""
+        naux, n, _ = self.tensor_info(slot)
+        nmo = c_spatial.shape[1]
+        c = _f64(c_spatial)
+        o, op = _out(out, (naux, nmo, nmo), np.float64)
+        self._call("rr_build_ri_nonrel_spin_mo_eri", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), c.ptr,
+                   _sz(nmo), op)
+        return o
+
+    def ri_nonrel_spin_mo_eri_rotated(self, b, u, precision=FP64, out=None):
+        """RiNonRelSpinMoEri::rotated -- src/NON_REL/NonRelSpinRiMoEri.cpp:45-95."""
+        naux, ns = b.shape[0], u.shape[0] // 2
+        bb, uu = _f64(b), _f64(u)
+        o, op = _out(out, (naux, ns, ns), np.float64)
+        self._call("rr_ri_nonrel_spin_mo_eri_rotated", ctypes.c_int(precision), _sz(ns), _sz(naux), bb.ptr, uu.ptr, op)
+        return o
+
+    def ri_nonrel_spin_embedded_b(self, b, out=None):
+        """RiNonRelSpinMoEri::embeddedB -- src/NON_REL/NonRelSpinRiMoEri.h:77-89."""
+        naux, ns = b.shape[0], b.shape[1]
+        bb = _f64(b)
+        o, op = _out(out, (naux, 2 * ns, 2 * ns), np.float64)
+        self._call("rr_ri_nonrel_spin_embedded_b", _sz(ns), _sz(naux), bb.ptr, op)
+        return o
+
+    # ---- Cholesky ------------------------------------------------------------------------------
+    def nonrel_fock_cholesky(self, slot, h_core, p, precision=FP64, out=None):
+        """nonRelFockMatrix(h, AoCholesky, P) -- src/Utils/AoCholesky.cpp:108-149."""
+        n = h_core.shape[0]
+        h, pp = _f64(h_core), _f64(p)
+        f, fp = _out(out, (n, n), np.float64)
+        self._call("rr_nonrel_fock_cholesky", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), h.ptr, pp.ptr, fp)
+        return f
+
+    def x2c_fock_cholesky(self, slot, h_x2c, p, precision=FP64, out=None):
+        """x2cFockMatrix(h, AoCholesky, P) -- src/Utils/AoCholesky.cpp:151-217."""
+        n2 = h_x2c.shape[0]
+        h, pp = _c128(h_x2c), _c128(p)
+        f, fp = _out(out, (n2, n2), np.complex128)
+        self._call("rr_x2c_fock_cholesky", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n2 // 2), h.ptr, pp.ptr,
+                   fp)
+        return f
+
+    def cholesky_to_mo_spatial(self, slot, c, precision=FP64, out=None):
+        """Spatial block of aoCholeskyToMoSpinOrbital -- src/Utils/AoCholesky.cpp:219-243."""
+        nchol, n, _ = self.tensor_info(slot)
+        nmo = c.shape[1]
+        cc = _f64(c)
+        o, op = _out(out, (nchol, nmo, nmo), np.float64)
+        self._call("rr_cholesky_to_mo_spatial", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), cc.ptr, _sz(nmo),
+                   op)
+        return o
+
+    def cholesky_to_mo_spinor(self, slot, c, precision=FP64, out=None):
+        """aoCholeskyToMoSpinor -- src/Utils/AoCholesky.cpp:270-295."""
+        nchol, n, _ = self.tensor_info(slot)
+        nmo = c.shape[1]
+        cc = _c128(c)
+        o, op = _out(out, (nchol, nmo, nmo), np.complex128)
+        self._call("rr_cholesky_to_mo_spinor", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), cc.ptr, _sz(nmo),
+                   op)
+        return o

@@ -1,0 +1,253 @@
+// The MFMA GEMM engine: C(m,n) = alpha * sum_k opA(A(m,k)) * opB(B(n,k)) [+ beta * C(m,n)].
+//
+// Every contraction of the RI hot path is written in this one "NT" form -- both operands stored
+// row-major with k contiguous -- by choosing the device layouts of the 3-index tensor and of its
+// half-transformed intermediate accordingly (rr_lib.hip). Operand fragments are loaded straight from
+// global memory (L2/MALL-resident panels) into registers in 32-byte (fp64/fp32) or 16-byte (bf16)
+// runs, so the kernel needs no LDS and no barriers; each 256-thread workgroup is 2x2 waves and each
+// wave owns a (16*WM) x (16*WN) tile of 16x16 matrix-core accumulators.
+//
+// Features: batched operands (batch strides), split-K into separate partial outputs, lower-triangle
+// tile scheduling for Hermitian rank-k updates, complex operands (interleaved re/im) handled as real
+// GEMMs with the sign/swap of the B fragment chosen by the conjugation mode, and an XCD-aware
+// workgroup order so that the tiles of one k-split share one XCD's L2.
+#pragma once
+
+#include "rr_device.hpp"
+
+namespace rr {
+
+struct GemmArgs {
+  const void* A = nullptr;
+  long long lda = 0, sAb = 0;  // logical elements (a complex value counts once)
+  const void* B = nullptr;
+  long long ldb = 0, sBb = 0;
+  void* C = nullptr;
+  long long ldcm = 0, ldcn = 1, sCb = 0, sCz = 0;
+  int M = 0, N = 0, K = 0;
+  int kper = 0;          // logical k per split (multiple of the k-slab)
+  int tiles_m = 0, tiles_n = 0, ntiles = 0;
+  int tri = 0;           // 1: only tiles with tile_m >= tile_n
+  int nbatch = 1, nsplit = 1;
+  int xcd_remap = 1;
+  double alpha = 1.0, beta = 0.0;
+  int conj = 0;          // bit0: conjugate A, bit1: conjugate B (complex only)
+  // Optional second output: rows m >= mcut (if mcut > 0) or columns n >= ncut (if ncut > 0) are
+  // written to C2 at (m - mcut, n) or (m, n - ncut) with the same strides.
+  void* C2 = nullptr;
+  int mcut = 0, ncut = 0;
+};
+
+template <typename TO>
+__device__ inline void store_out(TO* p, double v);
+template <>
+__device__ inline void store_out<double>(double* p, double v) {
+  *p = v;
+}
+template <>
+__device__ inline void store_out<float>(float* p, double v) {
+  *p = static_cast<float>(v);
+}
+template <>
+__device__ inline void store_out<bf16_t>(bf16_t* p, double v) {
+  *p = f32_to_bf16(static_cast<float>(v));
+}
+
+__device__ inline int remap_xcd(int w, int total) {
+  // Workgroup w runs on XCD (w % 8). Give XCD x a contiguous block of logical work items.
+  const int per = total >> 3, rem = total & 7, x = w & 7;
+  return x * per + (x < rem ? x : rem) + (w >> 3);
+}
+
+// Complex fragments hold interleaved (re, im) pairs.
+__device__ inline void cplx_conj(Mma<double>::frag& f) {
+  f.v[1] = -f.v[1];
+  f.v[3] = -f.v[3];
+}
+__device__ inline void cplx_conj(Mma<float>::frag& f) {
+#pragma unroll
+  for (int p = 1; p < 8; p += 2) f.v[p] = -f.v[p];
+}
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+__device__ inline void cplx_conj(Mma<bf16_t>::frag& f) {
+  us8 u = __builtin_bit_cast(us8, f.v);
+  u ^= us8{0, 0x8000, 0, 0x8000, 0, 0x8000, 0, 0x8000};
+  f.v = __builtin_bit_cast(bf8, u);
+}
+template <typename F>
+__device__ inline void cplx_split_pairs(const F& b, F& fr, F& fi, bool conj_b, int npairs) {
+#pragma unroll
+  for (int p = 0; p < npairs; ++p) {
+    const auto br = b.v[2 * p], bi = b.v[2 * p + 1];
+    fr.v[2 * p] = br;
+    fr.v[2 * p + 1] = conj_b ? bi : -bi;
+    fi.v[2 * p] = conj_b ? -bi : bi;
+    fi.v[2 * p + 1] = br;
+  }
+}
+__device__ inline void cplx_split(const Mma<double>::frag& b, Mma<double>::frag& fr, Mma<double>::frag& fi, bool cb) {
+  cplx_split_pairs(b, fr, fi, cb, 2);
+}
+__device__ inline void cplx_split(const Mma<float>::frag& b, Mma<float>::frag& fr, Mma<float>::frag& fi, bool cb) {
+  cplx_split_pairs(b, fr, fi, cb, 4);
+}
+__device__ inline void cplx_split(const Mma<bf16_t>::frag& b, Mma<bf16_t>::frag& fr, Mma<bf16_t>::frag& fi, bool cb) {
+  const us8 u = __builtin_bit_cast(us8, b.v);
+  const unsigned short S = 0x8000;
+  us8 r, i;
+  // fr = (br, cb ? bi : -bi), fi = (cb ? -bi : bi, br)
+  r = us8{u[0], (unsigned short)(u[1] ^ (cb ? 0 : S)), u[2], (unsigned short)(u[3] ^ (cb ? 0 : S)),
+          u[4], (unsigned short)(u[5] ^ (cb ? 0 : S)), u[6], (unsigned short)(u[7] ^ (cb ? 0 : S))};
+  i = us8{(unsigned short)(u[1] ^ (cb ? S : 0)), u[0], (unsigned short)(u[3] ^ (cb ? S : 0)), u[2],
+          (unsigned short)(u[5] ^ (cb ? S : 0)), u[4], (unsigned short)(u[7] ^ (cb ? S : 0)), u[6]};
+  fr.v = __builtin_bit_cast(bf8, r);
+  fi.v = __builtin_bit_cast(bf8, i);
+}
+
+template <typename TI, typename TO, int WM, int WN, bool CPLX>
+__global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
+  using MM = Mma<TI>;
+  using acc_t = typename MM::acc_t;
+  using frag = typename MM::frag;
+  constexpr int KL = MM::KL;
+  constexpr int SLAB = 4 * KL;  // real elements of k per super-step
+  constexpr int CW = CPLX ? 2 : 1;
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+
+  const int total = g.ntiles * g.nbatch * g.nsplit;
+  int w = static_cast<int>(blockIdx.x);
+  if (g.xcd_remap) w = remap_xcd(w, total);
+  const int tile = w % g.ntiles;
+  const int rest = w / g.ntiles;
+  const int batch = rest % g.nbatch;
+  const int split = rest / g.nbatch;
+
+  int tm, tn;
+  if (g.tri) {
+    int t = tile;
+    int a = static_cast<int>((sqrtf(8.0f * static_cast<float>(t) + 1.0f) - 1.0f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    while (a * (a + 1) / 2 > t) --a;
+    tm = a;
+    tn = t - a * (a + 1) / 2;
+  } else {
+    tm = tile % g.tiles_m;
+    tn = tile / g.tiles_m;
+  }
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, grp = lane >> 4;
+  const int m0 = tm * BM + (wave & 1) * (16 * WM);
+  const int n0 = tn * BN + (wave >> 1) * (16 * WN);
+
+  const long long kbeg = static_cast<long long>(split) * g.kper * CW;
+  long long kend = kbeg + static_cast<long long>(g.kper) * CW;
+  const long long ktot = static_cast<long long>(g.K) * CW;
+  if (kend > ktot) kend = ktot;
+
+  const TI* Ab = static_cast<const TI*>(g.A) + static_cast<long long>(batch) * g.sAb * CW;
+  const TI* Bb = static_cast<const TI*>(g.B) + static_cast<long long>(batch) * g.sBb * CW;
+  const TI* arow[WM];
+  const TI* brow[WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i) {
+    const int m = m0 + 16 * i + r;
+    arow[i] = (m < g.M) ? Ab + static_cast<long long>(m) * g.lda * CW : nullptr;
+  }
+#pragma unroll
+  for (int j = 0; j < WN; ++j) {
+    const int n = n0 + 16 * j + r;
+    brow[j] = (n < g.N) ? Bb + static_cast<long long>(n) * g.ldb * CW : nullptr;
+  }
+
+  acc_t acc[WM][WN];
+  acc_t acci[CPLX ? WM : 1][CPLX ? WN : 1];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = acc_t{};
+  if constexpr (CPLX) {
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j) acci[i][j] = acc_t{};
+  }
+
+  const bool ca = (g.conj & 1) != 0, cb = (g.conj & 2) != 0;
+  for (long long k = kbeg; k < kend; k += SLAB) {
+    const long long kk = k + grp * KL;
+    const bool kin = kk < kend;
+    frag fa[WM], fb[WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      if (kin && arow[i])
+        MM::load(fa[i], arow[i] + kk);
+      else
+        MM::zero(fa[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      if (kin && brow[j])
+        MM::load(fb[j], brow[j] + kk);
+      else
+        MM::zero(fb[j]);
+    }
+    if constexpr (!CPLX) {
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) MM::mma(acc[i][j], fa[i], fb[j]);
+    } else {
+      // Interleaved (re, im) pairs; conj(A) is applied to the A fragment itself, then
+      // Re C = a . bR and Im C = a . bI with, per pair (br, bi):
+      //   B plain: bR = (br, -bi), bI = ( bi, br)
+      //   conj(B): bR = (br,  bi), bI = (-bi, br)
+      if (ca) {
+#pragma unroll
+        for (int i = 0; i < WM; ++i) cplx_conj(fa[i]);
+      }
+#pragma unroll
+      for (int j = 0; j < WN; ++j) {
+        frag fr, fi;
+        cplx_split(fb[j], fr, fi, cb);
+#pragma unroll
+        for (int i = 0; i < WM; ++i) {
+          MM::mma(acc[i][j], fa[i], fr);
+          MM::mma(acci[i][j], fa[i], fi);
+        }
+      }
+    }
+  }
+
+  // Epilogue: lane owns rows out_row(grp, e) of column r in each 16x16 tile.
+  const long long coff = (static_cast<long long>(batch) * g.sCb + static_cast<long long>(split) * g.sCz) * CW;
+  TO* Cb = static_cast<TO*>(g.C) + coff;
+  TO* Cb2 = g.C2 ? static_cast<TO*>(g.C2) + coff : nullptr;
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int n = n0 + 16 * j + r;
+      if (n >= g.N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = m0 + 16 * i + MM::out_row(grp, e);
+        if (m >= g.M) continue;
+        int mm = m, nn = n;
+        TO* base = Cb;
+        if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
+        if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        TO* p = base + (static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn) * CW;
+        double v = g.alpha * static_cast<double>(acc[i][j][e]);
+        if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
+        store_out<TO>(p, v);
+        if constexpr (CPLX) {
+          double vi = g.alpha * static_cast<double>(acci[i][j][e]);
+          if (g.beta != 0.0) vi += g.beta * Conv<TO>::from(p[1]);
+          store_out<TO>(p + 1, vi);
+        }
+      }
+    }
+}
+
+}  // namespace rr

@@ -1,0 +1,1249 @@
+// librerdmft_gpu: MI355X (gfx950) RI / Cholesky J/K Fock builds and RI AO->MO transforms behind a
+// C ABI (include/rerdmft_gpu.h).
+//
+// Device layout of a 3-index tensor slot (an RI tensor (n_aux, n*n) or a set of Cholesky vectors):
+//   Bt[P][nu][mu] = B[P][mu][nu]   (each aux row transposed), leading dimension ld = roundup(n, 8),
+//   columns [n, ld) zero-filled, kept in any of fp64 / fp32 / bf16 (all copies resident in HBM).
+// With this layout every contraction of the path is the "NT" form C = A B^T of rr_gemm.hpp:
+//   exchange half-transform  W[nu][(P,i)] = sum_mu Bt_P[nu][mu] Lt[i][mu]      (A = Bt_P, B = Lt)
+//   exchange build           K[mu][nu]    = sum_(P,i) W[mu][(P,i)] W[nu][(P,i)] (Hermitian rank-k)
+//   Coulomb                  gamma_P = <Bt_P, P^T>,  Jt = sum_P gamma_P Bt_P      (HBM streams)
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rr_gemm.hpp"
+#include "../../include/rerdmft_gpu.h"
+
+namespace rr {
+
+// ================================================================================================
+// Context, errors, workspace
+// ================================================================================================
+struct Tensor {
+  size_t naux = 0, n = 0, ld = 0;
+  double* f64 = nullptr;
+  float* f32 = nullptr;
+  bf16_t* b16 = nullptr;
+  bool symmetric = true;
+};
+
+constexpr int kSlots = 16;
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::string err;
+  Tensor slot[kSlots];
+  std::map<int, std::pair<void*, size_t>> ws;
+  int tile_override = -1;
+  float last_ms[8] = {0};
+};
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define RR_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw Error{RR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)};             \
+  } while (0)
+
+#define RR_REQUIRE(cond, code, msg) \
+  do {                              \
+    if (!(cond)) throw Error{code, msg}; \
+  } while (0)
+
+inline size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// Named grow-only device workspace buffers.
+void* wsbuf(Context& c, int id, size_t bytes) {
+  auto it = c.ws.find(id);
+  if (it != c.ws.end() && it->second.second >= bytes) return it->second.first;
+  if (it != c.ws.end()) {
+    RR_HIP(hipStreamSynchronize(c.stream));
+    RR_HIP(hipFree(it->second.first));
+    c.ws.erase(it);
+  }
+  void* p = nullptr;
+  RR_HIP(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+  c.ws[id] = {p, std::max<size_t>(bytes, 256)};
+  return p;
+}
+
+enum WsId {
+  WS_STAGE = 1, WS_DT, WS_GPART, WS_GAMMA, WS_JPART, WS_LT, WS_W, WS_KPART, WS_OUT_F, WS_OUT_J, WS_OUT_K,
+  WS_H, WS_S, WS_X, WS_P, WS_T1, WS_T2, WS_T3, WS_T4, WS_EIGW, WS_EIGE, WS_INFO, WS_COCC, WS_W2, WS_CA, WS_CB,
+  WS_MO_A, WS_MO_B, WS_MO_S1, WS_MO_OUT, WS_DENS2, WS_TMP8, WS_TMP9
+};
+
+// True when p is device (or managed) memory visible to the current device.
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Copy host-or-device array into a device workspace buffer (returns device pointer).
+template <typename T>
+T* to_device(Context& c, int id, const T* src, size_t count) {
+  T* d = static_cast<T*>(wsbuf(c, id, count * sizeof(T)));
+  if (count) RR_HIP(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyDefault, c.stream));
+  return d;
+}
+template <typename T>
+void from_device(Context& c, T* dst, const T* src, size_t count) {
+  if (count) RR_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDefault, c.stream));
+}
+
+// ================================================================================================
+// Elementwise / layout kernels
+// ================================================================================================
+template <typename T>
+__device__ inline T from_double(double v) {
+  return Conv<T>::to(v);
+}
+
+// Transposed store of `rows` aux rows: out[P][nu][mu] = in[P][mu][nu], zero-fill columns [n, ld).
+template <typename TO>
+__global__ void k_store_rows(const double* in, size_t n, size_t ld, TO* out) {
+  __shared__ double tile[32][33];
+  const size_t P = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const size_t mu0 = blockIdx.x * 32, nu0 = blockIdx.y * 32;
+  const double* src = in + P * n * n;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const size_t mu = mu0 + yy, nu = nu0 + tx;
+    tile[yy][tx] = (mu < n && nu < n) ? src[mu * n + nu] : 0.0;
+  }
+  __syncthreads();
+  TO* dst = out + P * n * ld;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const size_t nu = nu0 + yy, mu = mu0 + tx;
+    if (nu < n && mu < ld) dst[nu * ld + mu] = from_double<TO>(tile[tx][yy]);
+  }
+}
+
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// Synthetic symmetric 3-index tensor: value(P, mu, nu) = sigma * N(0,1) * exp(-|mu-nu| / decay),
+// N(0,1) by Box-Muller from a counter-based hash of (seed, P, min(mu,nu), max(mu,nu)).
+template <typename TO>
+__global__ void k_generate(size_t naux, size_t n, size_t ld, double sigma, double decay, uint64_t seed, size_t p0,
+                           size_t prows, TO* out) {
+  const size_t total = prows * n * ld;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t P = p0 + idx / (n * ld);
+    const size_t rem = idx % (n * ld);
+    const size_t nu = rem / ld, mu = rem % ld;
+    double v = 0.0;
+    if (mu < n) {
+      const size_t a = mu < nu ? mu : nu, b = mu < nu ? nu : mu;
+      const uint64_t key = splitmix64(seed ^ splitmix64(((uint64_t)P * n + a) * n + b));
+      const uint64_t k2 = splitmix64(key);
+      const double u1 = ((key >> 11) + 1) * (1.0 / 9007199254740993.0);
+      const double u2 = (k2 >> 11) * (1.0 / 9007199254740992.0);
+      const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+      const double d = (double)(b - a);
+      v = sigma * z * exp(-d / decay);
+    }
+    out[(P - p0) * n * ld + rem] = from_double<TO>(v);
+  }
+}
+
+template <typename TI, typename TO>
+__global__ void k_convert(const TI* in, TO* out, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = from_double<TO>(Conv<TI>::from(in[i]));
+}
+
+// out[r][c] (ld_out) = scale * op(in)[r][c] for r < Rv, c < Cc (op = transpose when trans); rows
+// [Rv, R) and columns [Cc, ld_out) are zero-filled.
+template <typename TO>
+__global__ void k_pack(const double* in, size_t ld_in, int trans, size_t R, size_t Rv, size_t Cc, size_t ld_out,
+                       double scale, TO* out) {
+  const size_t total = R * ld_out;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = idx / ld_out, c = idx % ld_out;
+    double v = 0.0;
+    if (c < Cc && r < Rv) v = scale * (trans ? in[c * ld_in + r] : in[r * ld_in + c]);
+    out[idx] = from_double<TO>(v);
+  }
+}
+
+__global__ void k_conj_inplace(double* a, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    a[2 * i + 1] = -a[2 * i + 1];
+}
+
+__global__ void k_zero(double* p, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+inline int grid_for(size_t total, int block = 256, int maxg = 65536) {
+  size_t g = (total + block - 1) / block;
+  if (g < 1) g = 1;
+  return (int)std::min<size_t>(g, maxg);
+}
+
+// ================================================================================================
+// Coulomb streams
+// ================================================================================================
+template <typename TB>
+struct Vec16;  // 16-byte vector load, widened to double
+template <>
+struct Vec16<double> {
+  static constexpr int N = 2;
+  __device__ static void load(const double* p, double* o) {
+    d2 v = *reinterpret_cast<const d2*>(p);
+    o[0] = v.x, o[1] = v.y;
+  }
+};
+template <>
+struct Vec16<float> {
+  static constexpr int N = 4;
+  __device__ static void load(const float* p, double* o) {
+    f4 v = *reinterpret_cast<const f4*>(p);
+    o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w;
+  }
+};
+template <>
+struct Vec16<bf16_t> {
+  static constexpr int N = 8;
+  __device__ static void load(const bf16_t* p, double* o) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 v = *reinterpret_cast<const u4*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      union {
+        unsigned u;
+        float f;
+      } lo, hi;
+      lo.u = v[i] << 16;
+      hi.u = v[i] & 0xffff0000u;
+      o[2 * i] = lo.f, o[2 * i + 1] = hi.f;
+    }
+  }
+};
+
+__device__ inline double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+  return s;
+}
+
+// gpart[split][P] = sum_{e in split's chunk} Bt[P][e] * Dt[e], R aux rows per workgroup.
+template <typename TB, int R>
+__global__ __launch_bounds__(256) void k_gamma(const TB* B, size_t rowlen, size_t naux, const TB* Dt, size_t chunk,
+                                               double* gpart) {
+  using V = Vec16<TB>;
+  constexpr int VN = V::N;
+  __shared__ double sh[8];
+  const size_t P0 = (size_t)blockIdx.x * R;
+  const size_t split = blockIdx.y;
+  const size_t e0 = split * chunk;
+  const size_t e1 = std::min(rowlen, e0 + chunk);
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (size_t e = e0 + (size_t)threadIdx.x * VN; e < e1; e += 256 * VN) {
+    double dv[VN];
+    V::load(Dt + e, dv);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (P0 + r < naux) {
+        double bv[VN];
+        V::load(B + (P0 + r) * rowlen + e, bv);
+#pragma unroll
+        for (int q = 0; q < VN; ++q) acc[r] += bv[q] * dv[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double s = block_sum(acc[r], sh);
+    if (threadIdx.x == 0 && P0 + r < naux) gpart[split * naux + P0 + r] = s;
+  }
+}
+
+// gamma[P] = sum_split gpart[split][P] (+ gamma[P] when accumulate)
+__global__ void k_gamma_reduce(const double* gpart, int nsplit, size_t naux, double* gamma, int accumulate) {
+  for (size_t P = blockIdx.x * (size_t)blockDim.x + threadIdx.x; P < naux; P += (size_t)gridDim.x * blockDim.x) {
+    double s = accumulate ? gamma[P] : 0.0;
+    for (int z = 0; z < nsplit; ++z) s += gpart[(size_t)z * naux + P];
+    gamma[P] = s;
+  }
+}
+
+// jpart[split][e] = sum_{P in split range} gamma[P] * Bt[P][e]; each thread owns U 16-byte vectors.
+template <typename TB, int U>
+__global__ __launch_bounds__(256) void k_coulomb(const TB* B, size_t rowlen, size_t naux, const double* gamma,
+                                                 size_t p_per_split, double* jpart) {
+  using V = Vec16<TB>;
+  constexpr int VN = V::N;
+  const size_t base = (size_t)blockIdx.x * (256 * VN * U) + (size_t)threadIdx.x * VN;
+  const size_t split = blockIdx.y;
+  const size_t pb = split * p_per_split, pe = std::min(naux, pb + p_per_split);
+  double acc[U][VN];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int q = 0; q < VN; ++q) acc[u][q] = 0.0;
+  bool valid[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) valid[u] = base + (size_t)u * 256 * VN < rowlen;
+  for (size_t P = pb; P < pe; ++P) {
+    const double g = gamma[P];
+    const TB* row = B + P * rowlen;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (valid[u]) {
+        double bv[VN];
+        V::load(row + base + (size_t)u * 256 * VN, bv);
+#pragma unroll
+        for (int q = 0; q < VN; ++q) acc[u][q] += g * bv[q];
+      }
+    }
+  }
+  double* out = jpart + split * rowlen;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (valid[u]) {
+#pragma unroll
+      for (int q = 0; q < VN; ++q) out[base + (size_t)u * 256 * VN + q] = acc[u][q];
+    }
+}
+
+// ================================================================================================
+// Epilogues
+// ================================================================================================
+// Real NON_REL assembly: J[mu][nu] = sum_s Jt_s[nu*ld + mu]; K from split-K partials of a
+// lower-triangle-tiled (tri) or full (sym_sum: K = G + G^T) rank-k update; F = h + J + kfac*K.
+struct RealAsm {
+  size_t n = 0, ld = 0;
+  const double* jpart = nullptr;
+  int jsplit = 0;
+  size_t jstride = 0;
+  const double* kpart = nullptr;
+  int ksplit = 0;
+  size_t kstride = 0;
+  int ktile = 0;   // tile size of the tri layout (0: full G)
+  int ksym = 0;    // full G only: 1 -> K = G + G^T, 0 -> K = G
+  double kfac = -0.5;
+  const double* h = nullptr;
+  double* F = nullptr;
+  double* J = nullptr;
+  double* K = nullptr;
+};
+
+__global__ void k_real_asm(RealAsm a) {
+  const size_t total = a.n * a.n;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t mu = idx / a.n, nu = idx % a.n;
+    double j = 0.0;
+    if (a.jpart)
+      for (int s = 0; s < a.jsplit; ++s) j += a.jpart[s * a.jstride + nu * a.ld + mu];
+    double k = 0.0;
+    if (a.kpart) {
+      if (a.ktile > 0) {
+        const bool lower = (mu / a.ktile) >= (nu / a.ktile);
+        const size_t src = lower ? mu * a.n + nu : nu * a.n + mu;
+        for (int s = 0; s < a.ksplit; ++s) k += a.kpart[s * a.kstride + src];
+      } else if (a.ksym) {
+        for (int s = 0; s < a.ksplit; ++s) k += a.kpart[s * a.kstride + mu * a.n + nu] + a.kpart[s * a.kstride + nu * a.n + mu];
+      } else {
+        for (int s = 0; s < a.ksplit; ++s) k += a.kpart[s * a.kstride + mu * a.n + nu];
+      }
+    }
+    if (a.J) a.J[idx] = j;
+    if (a.K) a.K[idx] = k;
+    if (a.F) a.F[idx] = (a.h ? a.h[idx] : 0.0) + j + a.kfac * k;
+  }
+}
+
+// Complex Hermitian assembly over nb diagonal Coulomb blocks: out[a][b] (complex, dim nd) =
+//   h + J_blk (on the diagonal blocks listed in jblk) - K, K from tri-tiled complex partials.
+struct CplxAsm {
+  size_t nd = 0;
+  const double* kpart = nullptr;  // complex interleaved, partials [ksplit][nd][nd]
+  int ksplit = 0;
+  size_t kstride = 0;             // in complex elements
+  int ktile = 0;
+  double kfac = -1.0;
+  const double* h = nullptr;      // complex nd x nd (may be null)
+  double* out = nullptr;          // complex nd x nd
+  // Coulomb: up to 4 diagonal blocks; block b occupies [off[b], off[b]+sz[b]) and reads
+  // J = jt[...] from jpart (real, transposed layout with leading dimension jld[b])
+  int nblk = 0;
+  size_t off[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0}, jld[4] = {0, 0, 0, 0};
+  const double* jsrc[4] = {nullptr, nullptr, nullptr, nullptr};  // reduced Jt for each block
+};
+
+__global__ void k_cplx_asm(CplxAsm a) {
+  const size_t total = a.nd * a.nd;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = idx / a.nd, c = idx % a.nd;
+    double kr = 0.0, ki = 0.0;
+    if (a.kpart) {
+      const bool lower = (r / a.ktile) >= (c / a.ktile);
+      const size_t src = lower ? r * a.nd + c : c * a.nd + r;
+      for (int s = 0; s < a.ksplit; ++s) {
+        kr += a.kpart[2 * (s * a.kstride + src)];
+        ki += a.kpart[2 * (s * a.kstride + src) + 1];
+      }
+      if (!lower) ki = -ki;
+    }
+    double vr = a.kfac * kr, vi = a.kfac * ki;
+    for (int b = 0; b < a.nblk; ++b) {
+      if (r >= a.off[b] && r < a.off[b] + a.sz[b] && c >= a.off[b] && c < a.off[b] + a.sz[b]) {
+        const size_t mu = r - a.off[b], nu = c - a.off[b];
+        vr += a.jsrc[b][nu * a.jld[b] + mu];
+      }
+    }
+    if (a.h) vr += a.h[2 * idx], vi += a.h[2 * idx + 1];
+    a.out[2 * idx] = vr;
+    a.out[2 * idx + 1] = vi;
+  }
+}
+
+// Reduce split partials: out[e] = sum_s part[s*stride + e].
+__global__ void k_reduce_parts(const double* part, int nsplit, size_t stride, size_t count, double* out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += part[z * stride + i];
+    out[i] = s;
+  }
+}
+
+// ================================================================================================
+// GEMM dispatch (tile configurations instantiated for gfx950; the autotuner picks among them)
+// ================================================================================================
+enum Prec { P64 = 0, P32 = 1, P16 = 2 };
+
+struct TileCfg {
+  int wm, wn;
+};
+// Index -> (WM, WN); each workgroup tile is (32*WM) x (32*WN).
+static const TileCfg kTiles[] = {{1, 1}, {2, 2}, {4, 4}, {2, 4}, {4, 2}, {2, 7}, {1, 7}, {4, 1}};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+template <typename TI, typename TO, bool CPLX>
+void launch_gemm_cfg(int cfg, const GemmArgs& g, int grid, hipStream_t s) {
+  switch (cfg) {
+#define RR_CASE(i, WM, WN) \
+  case i:                  \
+    hipLaunchKernelGGL((gemm_nt_kernel<TI, TO, WM, WN, CPLX>), dim3(grid), dim3(256), 0, s, g); break;
+    RR_CASE(0, 1, 1)
+    RR_CASE(1, 2, 2)
+    RR_CASE(2, 4, 4)
+    RR_CASE(3, 2, 4)
+    RR_CASE(4, 4, 2)
+    RR_CASE(5, 2, 7)
+    RR_CASE(6, 1, 7)
+    RR_CASE(7, 4, 1)
+#undef RR_CASE
+    default:
+      throw Error{RR_ERR_ARG, "bad tile configuration"};
+  }
+}
+
+struct GemmSpec {
+  int prec = P64;      // input type
+  int out = P64;       // output type
+  bool cplx = false;
+  int cfg = 1;
+};
+
+void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
+  RR_REQUIRE(sp.cfg >= 0 && sp.cfg < kNumTiles, RR_ERR_ARG, "bad tile configuration");
+  const TileCfg t = kTiles[sp.cfg];
+  const int BM = 32 * t.wm, BN = 32 * t.wn;
+  g.tiles_m = (g.M + BM - 1) / BM;
+  g.tiles_n = (g.N + BN - 1) / BN;
+  if (g.tri) {
+    RR_REQUIRE(BM == BN && g.M == g.N, RR_ERR_ARG, "triangular tiling needs square tiles and output");
+    g.ntiles = g.tiles_m * (g.tiles_m + 1) / 2;
+  } else {
+    g.ntiles = g.tiles_m * g.tiles_n;
+  }
+  if (g.kper <= 0) g.kper = g.K;
+  const long long total = (long long)g.ntiles * g.nbatch * g.nsplit;
+  if (total == 0 || g.M == 0 || g.N == 0) return;
+  RR_REQUIRE(total < (1ll << 31), RR_ERR_ARG, "GEMM grid too large");
+  const int grid = (int)total;
+  const bool cp = sp.cplx;
+#define RR_DISPATCH(TI, TO)                                              \
+  do {                                                                   \
+    if (cp)                                                              \
+      launch_gemm_cfg<TI, TO, true>(sp.cfg, g, grid, c.stream);          \
+    else                                                                 \
+      launch_gemm_cfg<TI, TO, false>(sp.cfg, g, grid, c.stream);         \
+  } while (0)
+  if (sp.prec == P64 && sp.out == P64) RR_DISPATCH(double, double);
+  else if (sp.prec == P32 && sp.out == P32) RR_DISPATCH(float, float);
+  else if (sp.prec == P32 && sp.out == P64) RR_DISPATCH(float, double);
+  else if (sp.prec == P16 && sp.out == P16) RR_DISPATCH(bf16_t, bf16_t);
+  else if (sp.prec == P16 && sp.out == P32) RR_DISPATCH(bf16_t, float);
+  else if (sp.prec == P16 && sp.out == P64) RR_DISPATCH(bf16_t, double);
+  else throw Error{RR_ERR_ARG, "unsupported GEMM precision combination"};
+#undef RR_DISPATCH
+  RR_HIP(hipGetLastError());
+}
+
+size_t prec_size(int p) { return p == P64 ? 8 : (p == P32 ? 4 : 2); }
+
+const void* tensor_data(const Tensor& t, int prec) {
+  const void* p = prec == P64 ? (const void*)t.f64 : (prec == P32 ? (const void*)t.f32 : (const void*)t.b16);
+  RR_REQUIRE(p != nullptr, RR_ERR_ARG, "tensor slot has no copy in the requested precision");
+  return p;
+}
+
+// ================================================================================================
+// Small FP64 products on device (recovery, embeddings): C = op(A) op(B), any op in {N, T, C}.
+// Operands are packed into k-contiguous zero-padded buffers so the NT engine applies.
+// ================================================================================================
+__global__ void k_pack_op(const double* in, size_t ld_in, char op, int cplx, size_t R, size_t K, size_t ld_out,
+                          double* out) {
+  // out row r (length ld_out) = op(in) row r, i.e. for op N: in[r][k]; for T/C: in[k][r].
+  const size_t total = R * ld_out;
+  const int cw = cplx ? 2 : 1;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = idx / ld_out, k = idx % ld_out;
+    double re = 0.0, im = 0.0;
+    if (k < K) {
+      const size_t src = (op == 'N') ? r * ld_in + k : k * ld_in + r;
+      re = in[cw * src];
+      if (cplx) im = in[cw * src + 1];
+      if (op == 'C') im = -im;
+    }
+    out[cw * idx] = re;
+    if (cplx) out[cw * idx + 1] = im;
+  }
+}
+
+// C (m x n, row-major, ldc) = op(A) (m x k) * op(B) (k x n); A/B device pointers with their own
+// row-major leading dimensions. cplx: interleaved complex.
+void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, size_t k, const double* A, size_t lda,
+                 const double* B, size_t ldb, double* C, size_t ldc, double alpha = 1.0, double beta = 0.0) {
+  if (m == 0 || n == 0) return;
+  const int cw = cplx ? 2 : 1;
+  const size_t kp = roundup(std::max<size_t>(k, 1), 8);
+  double* Ap = static_cast<double*>(wsbuf(c, WS_TMP8, m * kp * cw * sizeof(double)));
+  double* Bp = static_cast<double*>(wsbuf(c, WS_TMP9, n * kp * cw * sizeof(double)));
+  // A operand rows m of op(A); B operand rows n of op(B)^T: op(B)^T row j = column j of op(B).
+  hipLaunchKernelGGL(k_pack_op, dim3(grid_for(m * kp)), dim3(256), 0, c.stream, A, lda, opa, (int)cplx, m, k, kp, Ap);
+  // op(B)^T: if opb == 'N', row j of B^T = column j of B -> read in[k][j] => use op 'T'; 'T' -> read in[j][k] => 'N';
+  // 'C' -> conj of in[j][k] (conj, no transpose) -> handled as 'N' + conj flag below.
+  const char pb = (opb == 'N') ? 'T' : 'N';
+  hipLaunchKernelGGL(k_pack_op, dim3(grid_for(n * kp)), dim3(256), 0, c.stream, B, ldb, pb, (int)cplx, n, k, kp, Bp);
+  RR_HIP(hipGetLastError());
+  GemmArgs g;
+  g.A = Ap, g.lda = kp, g.B = Bp, g.ldb = kp, g.C = C, g.ldcm = ldc, g.ldcn = 1;
+  g.M = (int)m, g.N = (int)n, g.K = (int)kp, g.alpha = alpha, g.beta = beta;
+  g.conj = (cplx && opb == 'C') ? 2 : 0;
+  GemmSpec sp;
+  sp.prec = P64, sp.out = P64, sp.cplx = cplx, sp.cfg = (m >= 256 && n >= 256) ? 2 : 1;
+  run_gemm(c, sp, g);
+}
+
+// Symmetric / Hermitian eigensolve in place (row-major). On return, row j of `a` holds eigenvector
+// j (components along the row) and w the eigenvalues in ascending order.
+void eig_device(Context& c, bool cplx, size_t n, double* a, double* w) {
+  if (n == 0) return;
+  double* e = static_cast<double*>(wsbuf(c, WS_EIGE, n * sizeof(double)));
+  rocblas_int* info = static_cast<rocblas_int*>(wsbuf(c, WS_INFO, sizeof(rocblas_int)));
+  rocblas_status st;
+  // Row-major symmetric == column-major; a Hermitian row-major array read column-major is conj(A),
+  // whose eigenvectors are conj(v): row j of the column-major result is then conj(v_j) -> conjugate back.
+  if (!cplx)
+    st = rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, a, (rocblas_int)n, w, e, info);
+  else
+    st = rocsolver_zheevd(c.blas, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n,
+                          reinterpret_cast<rocblas_double_complex*>(a), (rocblas_int)n, w, e, info);
+  RR_REQUIRE(st == rocblas_status_success, RR_ERR_SOLVER, "rocsolver eigensolver failed");
+  rocblas_int hinfo = 0;
+  RR_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  RR_REQUIRE(hinfo == 0, RR_ERR_SOLVER, cplx ? "diagonalizeHermitian: eigensolver failed to converge"
+                                              : "diagonalizeSymmetric: eigensolver failed to converge");
+  if (cplx) {
+    // conjugate imaginary parts
+    hipLaunchKernelGGL(k_conj_inplace, dim3(grid_for(n * n)), dim3(256), 0, c.stream, a, n * n);
+    RR_HIP(hipGetLastError());
+  }
+}
+
+
+// ================================================================================================
+// Phase timing (HIP events on the library stream)
+// ================================================================================================
+struct PhaseTimer {
+  Context& c;
+  int phase;
+  hipEvent_t a = nullptr, b = nullptr;
+  static bool enabled;
+  static double ms[8];
+  static long long cnt[8];
+  PhaseTimer(Context& cc, int ph) : c(cc), phase(ph) {
+    if (!enabled) return;
+    RR_HIP(hipEventCreate(&a));
+    RR_HIP(hipEventCreate(&b));
+    RR_HIP(hipEventRecord(a, c.stream));
+  }
+  ~PhaseTimer() {
+    if (!enabled || !a) return;
+    if (hipEventRecord(b, c.stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, a, b) == hipSuccess) {
+        ms[phase] += t;
+        cnt[phase] += 1;
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+bool PhaseTimer::enabled = false;
+double PhaseTimer::ms[8] = {0};
+long long PhaseTimer::cnt[8] = {0};
+enum { PH_GAMMA = 0, PH_COUL = 1, PH_GEMM1 = 2, PH_GEMM2 = 3, PH_ASM = 4, PH_RECOVER = 5 };
+
+// ================================================================================================
+// Coulomb and exchange cores
+// ================================================================================================
+template <typename TB>
+TB* tensor_ptr(const Tensor& t);
+template <>
+double* tensor_ptr<double>(const Tensor& t) { return t.f64; }
+template <>
+float* tensor_ptr<float>(const Tensor& t) { return t.f32; }
+template <>
+bf16_t* tensor_ptr<bf16_t>(const Tensor& t) { return t.b16; }
+
+// gamma[P] (+)= <Bt_P, Dt> where Dt (n x ld, device, type of the tensor copy) is D^T padded.
+template <typename TB>
+void gamma_stream(Context& c, const Tensor& t, const TB* dDt, double* gamma, bool accumulate) {
+  const TB* B = tensor_ptr<TB>(t);
+  RR_REQUIRE(B, RR_ERR_STATE, "tensor slot has no copy in the requested precision");
+  const size_t rowlen = t.n * t.ld;
+  constexpr int R = 8;
+  const size_t nrb = (t.naux + R - 1) / R;
+  // Split the row so that the grid has >= ~2048 workgroups.
+  size_t nsplit = std::max<size_t>(1, std::min<size_t>(64, (2048 + nrb - 1) / nrb));
+  const size_t vn = 16 / sizeof(TB);
+  size_t chunk = roundup((rowlen + nsplit - 1) / nsplit, 256 * vn);
+  nsplit = (rowlen + chunk - 1) / chunk;
+  double* gpart = static_cast<double*>(wsbuf(c, WS_GPART, nsplit * t.naux * sizeof(double)));
+  {
+    PhaseTimer pt(c, PH_GAMMA);
+    hipLaunchKernelGGL((k_gamma<TB, R>), dim3((unsigned)nrb, (unsigned)nsplit), dim3(256), 0, c.stream, B, rowlen,
+                       t.naux, dDt, chunk, gpart);
+    RR_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_gamma_reduce, dim3(grid_for(t.naux)), dim3(256), 0, c.stream, gpart, (int)nsplit, t.naux, gamma,
+                     accumulate ? 1 : 0);
+  RR_HIP(hipGetLastError());
+}
+
+// jpart[s][e] partial sums of Jt = sum_P gamma_P Bt_P; returns the number of splits.
+template <typename TB>
+int coulomb_stream(Context& c, const Tensor& t, const double* gamma, double* jpart_out_or_null, double** jpart_used) {
+  const TB* B = tensor_ptr<TB>(t);
+  RR_REQUIRE(B, RR_ERR_STATE, "tensor slot has no copy in the requested precision");
+  const size_t rowlen = t.n * t.ld;
+  constexpr int U = 4;
+  const size_t vn = 16 / sizeof(TB);
+  const size_t per_wg = 256 * vn * U;
+  const size_t nx = (rowlen + per_wg - 1) / per_wg;
+  size_t nsplit = std::max<size_t>(1, std::min<size_t>(32, (2048 + nx - 1) / nx));
+  nsplit = std::min(nsplit, std::max<size_t>(1, t.naux));
+  const size_t pps = (t.naux + nsplit - 1) / nsplit;
+  nsplit = (t.naux + pps - 1) / pps;
+  double* jpart = jpart_out_or_null ? jpart_out_or_null
+                                    : static_cast<double*>(wsbuf(c, WS_JPART, std::max<size_t>(1, nsplit) * rowlen * sizeof(double)));
+  {
+    PhaseTimer pt(c, PH_COUL);
+    hipLaunchKernelGGL((k_coulomb<TB, U>), dim3((unsigned)nx, (unsigned)nsplit), dim3(256), 0, c.stream, B, rowlen, t.naux,
+                       gamma, pps, jpart);
+    RR_HIP(hipGetLastError());
+  }
+  *jpart_used = jpart;
+  return (int)nsplit;
+}
+
+int gemm1_cfg(Context& c, size_t ncols) {
+  if (c.tile_override >= 0) return c.tile_override;
+  if (ncols <= 32) return 0;   // 32 x 32
+  if (ncols <= 64) return 1;   // 64 x 64
+  if (ncols <= 128) return 4;  // 128 x 64 -> (4,2): BM=128, BN=64
+  return 5;                    // 64 x 224
+}
+int gemm2_cfg(Context& c, size_t n) {
+  if (c.tile_override >= 0 && kTiles[c.tile_override].wm == kTiles[c.tile_override].wn) return c.tile_override;
+  if (n <= 64) return 1;
+  return 2;  // 128 x 128
+}
+
+// Number of k-splits of the rank-k exchange GEMM: a function of the problem shape only, so that
+// the tile configuration never changes the summation order of any element.
+int gemm2_splits(size_t n, size_t kdim, int slab) {
+  const size_t t = (n + 127) / 128;
+  const size_t tiles = std::max<size_t>(1, t * (t + 1) / 2);
+  size_t ns = std::max<size_t>(1, (2048 + tiles - 1) / tiles);
+  ns = std::min<size_t>(ns, std::max<size_t>(1, kdim / (size_t)(slab * 8)));
+  return (int)std::min<size_t>(ns, 256);
+}
+
+struct ExchangeOut {
+  double* kpart = nullptr;
+  int ksplit = 0;
+  size_t kstride = 0;
+  int ktile = 0;  // tri tile size (0: full G)
+  int ksym = 0;   // full G: assemble G + G^T
+};
+
+// Real exchange in factor form on one tensor: W = Bt_P Lt^T (and W2 = Bt_P Rt^T), then
+// K partials = W W^T (tri) or W W2^T (full). Lt/Rt: device fp64, r x ld_l.
+template <typename TI>
+ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
+                          size_t ld_l) {
+  ExchangeOut out;
+  const size_t n = t.n, ld = t.ld, naux = t.naux;
+  const size_t rp = roundup(std::max<size_t>(r, 1), 8);
+  const int nfac = dRt ? 2 : 1;
+  // Pack the factor rows (zero-padded to rp rows and ld columns) in the operand type.
+  TI* lt = static_cast<TI*>(wsbuf(c, WS_LT, nfac * rp * ld * sizeof(TI)));
+  hipLaunchKernelGGL((k_pack<TI>), dim3(grid_for(rp * ld)), dim3(256), 0, c.stream, dLt, ld_l, 0, rp, r, n, ld, 1.0, lt);
+  if (dRt)
+    hipLaunchKernelGGL((k_pack<TI>), dim3(grid_for(rp * ld)), dim3(256), 0, c.stream, dRt, ld_l, 0, rp, r, n, ld, 1.0,
+                       lt + rp * ld);
+  RR_HIP(hipGetLastError());
+  const size_t kdim = naux * rp;  // flat k of the rank-k update
+  TI* W = static_cast<TI*>(wsbuf(c, WS_W, nfac * n * kdim * sizeof(TI)));
+  TI* W2 = dRt ? W + n * kdim : nullptr;
+  {
+    PhaseTimer pt(c, PH_GEMM1);
+    GemmArgs g;
+    g.A = tensor_data(t, prec);
+    g.lda = ld, g.sAb = n * ld;
+    g.B = lt, g.ldb = ld, g.sBb = 0;
+    g.C = W, g.ldcm = kdim, g.ldcn = 1, g.sCb = rp;
+    g.M = (int)n, g.N = (int)(nfac * rp), g.K = (int)ld, g.nbatch = (int)naux;
+    if (dRt) g.C2 = W2, g.ncut = (int)rp;
+    GemmSpec sp;
+    sp.prec = prec, sp.out = prec, sp.cfg = gemm1_cfg(c, nfac * rp);
+    run_gemm(c, sp, g);
+  }
+  const int slab = prec == P64 ? 16 : 32;
+  const int ns = gemm2_splits(n, kdim, slab);
+  size_t kper = roundup((kdim + ns - 1) / ns, slab);
+  const int nsplit = (int)((kdim + kper - 1) / kper);
+  double* kpart = static_cast<double*>(wsbuf(c, WS_KPART, (size_t)nsplit * n * n * sizeof(double)));
+  {
+    PhaseTimer pt(c, PH_GEMM2);
+    GemmArgs g;
+    g.A = W, g.lda = kdim;
+    g.B = dRt ? W2 : W, g.ldb = kdim;
+    g.C = kpart, g.ldcm = n, g.ldcn = 1, g.sCz = n * n;
+    g.M = (int)n, g.N = (int)n, g.K = (int)kdim, g.kper = (int)kper, g.nsplit = nsplit;
+    g.tri = dRt ? 0 : 1;
+    GemmSpec sp;
+    sp.prec = prec, sp.out = P64, sp.cfg = gemm2_cfg(c, n);
+    run_gemm(c, sp, g);
+    out.ktile = dRt ? 0 : 32 * kTiles[sp.cfg].wm;
+  }
+  out.kpart = kpart, out.ksplit = nsplit, out.kstride = n * n;
+  return out;
+}
+
+// J partial sums for a real density D (device fp64, n x n): returns jpart and split count.
+template <typename TB>
+int coulomb_real(Context& c, const Tensor& t, const double* dD, double** jpart) {
+  TB* dt = static_cast<TB*>(wsbuf(c, WS_DT, t.n * t.ld * sizeof(TB)));
+  hipLaunchKernelGGL((k_pack<TB>), dim3(grid_for(t.n * t.ld)), dim3(256), 0, c.stream, dD, t.n, 1, t.n, t.n, t.n, t.ld, 1.0,
+                     dt);
+  RR_HIP(hipGetLastError());
+  double* gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
+  gamma_stream<TB>(c, t, dt, gamma, false);
+  return coulomb_stream<TB>(c, t, gamma, nullptr, jpart);
+}
+
+int coulomb_real_prec(Context& c, const Tensor& t, int prec, const double* dD, double** jpart) {
+  if (prec == P64) return coulomb_real<double>(c, t, dD, jpart);
+  if (prec == P32) return coulomb_real<float>(c, t, dD, jpart);
+  return coulomb_real<bf16_t>(c, t, dD, jpart);
+}
+
+ExchangeOut exchange_real_prec(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
+                               size_t ld_l) {
+  if (prec == P64) return exchange_real<double>(c, t, prec, dLt, dRt, r, ld_l);
+  if (prec == P32) return exchange_real<float>(c, t, prec, dLt, dRt, r, ld_l);
+  return exchange_real<bf16_t>(c, t, prec, dLt, dRt, r, ld_l);
+}
+
+void real_assemble(Context& c, const Tensor& t, double* jpart, int jsplit, const ExchangeOut* ex, double kfac,
+                   const double* dh, double* dF, double* dJ, double* dK) {
+  PhaseTimer pt(c, PH_ASM);
+  RealAsm a;
+  a.n = t.n, a.ld = t.ld;
+  a.jpart = jpart, a.jsplit = jsplit, a.jstride = t.n * t.ld;
+  if (ex && ex->kpart)
+    a.kpart = ex->kpart, a.ksplit = ex->ksplit, a.kstride = ex->kstride, a.ktile = ex->ktile, a.ksym = ex->ksym;
+  a.kfac = kfac, a.h = dh, a.F = dF, a.J = dJ, a.K = dK;
+  hipLaunchKernelGGL(k_real_asm, dim3(grid_for(t.n * t.n)), dim3(256), 0, c.stream, a);
+  RR_HIP(hipGetLastError());
+}
+
+Tensor& get_slot(Context& c, int slot) {
+  RR_REQUIRE(slot >= 0 && slot < kSlots, RR_ERR_ARG, "tensor slot out of range");
+  Tensor& t = c.slot[slot];
+  RR_REQUIRE(t.f64 || t.f32 || t.b16, RR_ERR_STATE, "tensor slot is empty");
+  return t;
+}
+
+void check_prec(int prec) { RR_REQUIRE(prec == P64 || prec == P32 || prec == P16, RR_ERR_ARG, "bad precision"); }
+
+// Output helper: device pointer if the caller's buffer is device memory, else a workspace buffer
+// that is copied back by finish_out().
+struct OutBuf {
+  double* dev = nullptr;
+  double* user = nullptr;
+  size_t count = 0;
+  bool copy = false;
+};
+OutBuf out_buf(Context& c, int id, double* user, size_t count) {
+  OutBuf o;
+  o.user = user, o.count = count;
+  if (!user) return o;
+  if (is_device_ptr(user)) {
+    o.dev = user;
+  } else {
+    o.dev = static_cast<double*>(wsbuf(c, id, std::max<size_t>(1, count) * sizeof(double)));
+    o.copy = true;
+  }
+  return o;
+}
+void finish_out(Context& c, const OutBuf& o) {
+  if (o.copy && o.count) RR_HIP(hipMemcpyAsync(o.user, o.dev, o.count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+const double* in_buf(Context& c, int id, const double* user, size_t count) {
+  if (!user) return nullptr;
+  if (is_device_ptr(user)) return user;
+  return to_device<double>(c, id, user, count);
+}
+
+
+// Occupied-orbital recovery (NonRelFockMatrixRi.cpp:51-64, X2C_FockMatrixRi.cpp:65-95,
+// UkbFockMatrixRi.cpp:192-209): eigendecompose Peff = X^H (S P S) X * scale (FP64, rocSOLVER),
+// keep eigenvalues > 0.5 (the top of the ascending spectrum). Returns n_occ; on return
+// `rows` points at n_occ eigenvector rows (length nx, complex when cplx) and `occ_host` holds
+// their eigenvalues.
+size_t recover_occupied(Context& c, bool cplx, size_t n, size_t nx, const double* dS, const double* dP, const double* dX,
+                        double scale, const double** rows, std::vector<double>& occ_host) {
+  PhaseTimer pt(c, PH_RECOVER);
+  const int cw = cplx ? 2 : 1;
+  const char T = cplx ? 'C' : 'T';
+  double* t1 = static_cast<double*>(wsbuf(c, WS_T1, n * n * cw * sizeof(double)));
+  double* t2 = static_cast<double*>(wsbuf(c, WS_T2, n * n * cw * sizeof(double)));
+  double* t3 = static_cast<double*>(wsbuf(c, WS_T3, n * std::max<size_t>(nx, 1) * cw * sizeof(double)));
+  double* t4 = static_cast<double*>(wsbuf(c, WS_T4, std::max<size_t>(nx * nx, 1) * cw * sizeof(double)));
+  dgemm_small(c, cplx, 'N', 'N', n, n, n, dS, n, dP, n, t1, n);                 // S P
+  dgemm_small(c, cplx, 'N', 'N', n, n, n, t1, n, dS, n, t2, n, scale);          // S P S * scale
+  dgemm_small(c, cplx, 'N', 'N', n, nx, n, t2, n, dX, nx, t3, nx);              // (SPS) X
+  dgemm_small(c, cplx, T, 'N', nx, nx, n, dX, nx, t3, nx, t4, nx);              // X^H (SPS) X
+  double* w = static_cast<double*>(wsbuf(c, WS_EIGW, std::max<size_t>(nx, 1) * sizeof(double)));
+  eig_device(c, cplx, nx, t4, w);
+  std::vector<double> wh(nx);
+  RR_HIP(hipMemcpyAsync(wh.data(), w, nx * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  size_t first = nx;
+  while (first > 0 && wh[first - 1] > 0.5) --first;
+  occ_host.assign(wh.begin() + first, wh.end());
+  *rows = t4 + first * nx * cw;
+  return nx - first;
+}
+
+__global__ void k_scale_rows_sqrt(double* a, size_t rows, size_t cols, int cw, const double* occ) {
+  const size_t total = rows * cols * cw;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+    a[i] *= sqrt(occ[i / (cols * cw)]);
+}
+
+
+// c = a*x + b*y elementwise
+__global__ void k_axpby(size_t count, double a, const double* x, double b, const double* y, double* out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = a * x[i] + b * y[i];
+}
+// out = 1.5 X - 0.5 X (X^T X) helper pieces: Y = 1.5*X - 0.5*Z
+// ||X^T X - I||_max into *err (single workgroup)
+__global__ void k_orth_err(const double* xtx, size_t m, double* err) {
+  __shared__ double sh[256];
+  double e = 0.0;
+  for (size_t i = threadIdx.x; i < m * m; i += blockDim.x) {
+    const double v = xtx[i] - ((i / m) == (i % m) ? 1.0 : 0.0);
+    e = fmax(e, fabs(v));
+  }
+  sh[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *err = sh[0];
+}
+
+// Incremental build for D = 2 C C^T between two orbital sets (rr_ri_jk_delta):
+//   Q = polar(C_new^T C_old) (Newton-Schulz, FP64), C' = C_new Q (same density as C_new, aligned to
+//   C_old), dC = C' - C_old, M = C_old + dC/2, so D_new - D_old = 2 (M dC^T + dC M^T) exactly;
+//   dJ = J(dD); dK = G + G^T with G = sum_P (B_P^T sqrt2 M)(B_P^T sqrt2 dC)^T.
+void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, const double* dA, const double* dB,
+                 double* dJ, double* dK) {
+  if (no == 0) {
+    if (dJ) hipLaunchKernelGGL(k_zero, dim3(grid_for(n * n)), dim3(256), 0, c.stream, dJ, n * n);
+    if (dK) hipLaunchKernelGGL(k_zero, dim3(grid_for(n * n)), dim3(256), 0, c.stream, dK, n * n);
+    return;
+  }
+  double* q = static_cast<double*>(wsbuf(c, WS_T1, no * no * sizeof(double)));
+  double* z = static_cast<double*>(wsbuf(c, WS_T2, no * no * sizeof(double)));
+  double* y = static_cast<double*>(wsbuf(c, WS_T3, no * no * sizeof(double)));
+  double* err = static_cast<double*>(wsbuf(c, WS_TMP8 + 100, sizeof(double)));
+  dgemm_small(c, false, 'T', 'N', no, no, n, dA, no, dB, no, q, no);  // C_new^T C_old
+  for (int it = 0; it < 60; ++it) {
+    dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
+    if (it % 3 == 0) {
+      hipLaunchKernelGGL(k_orth_err, dim3(1), dim3(256), 0, c.stream, z, no, err);
+      double e = 0;
+      RR_HIP(hipMemcpyAsync(&e, err, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      RR_HIP(hipStreamSynchronize(c.stream));
+      if (e < 1e-14) break;
+      RR_REQUIRE(std::isfinite(e), RR_ERR_SOLVER, "rr_ri_jk_delta: orbital alignment diverged");
+    }
+    dgemm_small(c, false, 'N', 'N', no, no, no, q, no, z, no, y, no);  // X (X^T X)
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.5, q, -0.5, y, q);
+  }
+  double* cp = static_cast<double*>(wsbuf(c, WS_T4, n * no * sizeof(double)));
+  dgemm_small(c, false, 'N', 'N', n, no, no, dA, no, q, no, cp, no);  // C' = C_new Q
+  double* dc = static_cast<double*>(wsbuf(c, WS_COCC, 2 * n * no * sizeof(double)));
+  double* m = dc + n * no;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * no)), dim3(256), 0, c.stream, n * no, 1.0, cp, -1.0, dB, dc);   // dC
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * no)), dim3(256), 0, c.stream, n * no, 1.0, dB, 0.5, dc, m);     // M
+  RR_HIP(hipGetLastError());
+  double* jpart = nullptr;
+  int jsplit = 0;
+  if (dJ) {
+    double* dd = static_cast<double*>(wsbuf(c, WS_DENS2, n * n * sizeof(double)));
+    dgemm_small(c, false, 'N', 'T', n, n, no, m, no, dc, no, dd, n, 2.0);             // 2 M dC^T
+    dgemm_small(c, false, 'N', 'T', n, n, no, dc, no, m, no, dd, n, 2.0, 1.0);        // + 2 dC M^T
+    jsplit = coulomb_real_prec(c, t, prec, dd, &jpart);
+  }
+  ExchangeOut ex;
+  if (dK) {
+    double* lt = static_cast<double*>(wsbuf(c, WS_T1 + 200, 2 * no * n * sizeof(double)));
+    hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(no * n)), dim3(256), 0, c.stream, m, no, 1, no, no, n, n,
+                       std::sqrt(2.0), lt);
+    hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(no * n)), dim3(256), 0, c.stream, dc, no, 1, no, no, n, n,
+                       std::sqrt(2.0), lt + no * n);
+    RR_HIP(hipGetLastError());
+    ex = exchange_real_prec(c, t, prec, lt, lt + no * n, no, n);
+    ex.ksym = 1;
+  }
+  real_assemble(c, t, jpart, jsplit, dK ? &ex : nullptr, 0.0, nullptr, nullptr, dJ, dK);
+}
+
+}  // namespace rr
+
+// ================================================================================================
+// C ABI
+// ================================================================================================
+using rr::Context;
+using rr::Error;
+
+struct rr_context {
+  Context ctx;
+};
+
+namespace {
+template <typename F>
+int guarded(rr_handle h, F&& f) {
+  if (!h) return RR_ERR_ARG;
+  try {
+    RR_HIP(hipSetDevice(h->ctx.device));
+    f(h->ctx);
+    RR_HIP(hipStreamSynchronize(h->ctx.stream));
+    h->ctx.err.clear();
+    return RR_OK;
+  } catch (const Error& e) {
+    h->ctx.err = e.msg;
+    (void)hipStreamSynchronize(h->ctx.stream);
+    return e.code;
+  } catch (const std::exception& e) {
+    h->ctx.err = e.what();
+    return RR_ERR_INTERNAL;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int rr_create(int device, rr_handle* out) {
+  if (!out) return RR_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count) return RR_ERR_HIP;
+  rr_context* h = new rr_context();
+  h->ctx.device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->ctx.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return RR_ERR_HIP;
+  }
+  if (rocblas_create_handle(&h->ctx.blas) != rocblas_status_success ||
+      rocblas_set_stream(h->ctx.blas, h->ctx.stream) != rocblas_status_success) {
+    (void)hipStreamDestroy(h->ctx.stream);
+    delete h;
+    return RR_ERR_HIP;
+  }
+  *out = h;
+  return RR_OK;
+}
+
+int rr_destroy(rr_handle h) {
+  if (!h) return RR_ERR_ARG;
+  (void)hipSetDevice(h->ctx.device);
+  (void)hipStreamSynchronize(h->ctx.stream);
+  for (auto& t : h->ctx.slot) {
+    if (t.f64) (void)hipFree(t.f64);
+    if (t.f32) (void)hipFree(t.f32);
+    if (t.b16) (void)hipFree(t.b16);
+  }
+  for (auto& kv : h->ctx.ws) (void)hipFree(kv.second.first);
+  if (h->ctx.blas) rocblas_destroy_handle(h->ctx.blas);
+  (void)hipStreamDestroy(h->ctx.stream);
+  delete h;
+  return RR_OK;
+}
+
+const char* rr_last_error(rr_handle h) { return h ? h->ctx.err.c_str() : "null handle"; }
+
+int rr_set_tile_config(rr_handle h, int cfg) {
+  if (!h) return RR_ERR_ARG;
+  if (cfg < -1 || cfg >= rr::kNumTiles) {
+    h->ctx.err = "tile configuration out of range";
+    return RR_ERR_ARG;
+  }
+  h->ctx.tile_override = cfg;
+  return RR_OK;
+}
+
+int rr_timing(rr_handle h, int enable, int reset, double* ms, long long* counts, int n) {
+  if (!h) return RR_ERR_ARG;
+  for (int i = 0; i < n && i < 8; ++i) {
+    if (ms) ms[i] = rr::PhaseTimer::ms[i];
+    if (counts) counts[i] = rr::PhaseTimer::cnt[i];
+  }
+  if (reset)
+    for (int i = 0; i < 8; ++i) rr::PhaseTimer::ms[i] = 0.0, rr::PhaseTimer::cnt[i] = 0;
+  rr::PhaseTimer::enabled = enable != 0;
+  return RR_OK;
+}
+
+int rr_tensor_release(rr_handle h, int slot) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(slot >= 0 && slot < rr::kSlots, RR_ERR_ARG, "tensor slot out of range");
+    rr::Tensor& t = c.slot[slot];
+    RR_HIP(hipStreamSynchronize(c.stream));
+    if (t.f64) RR_HIP(hipFree(t.f64));
+    if (t.f32) RR_HIP(hipFree(t.f32));
+    if (t.b16) RR_HIP(hipFree(t.b16));
+    t = rr::Tensor();
+  });
+}
+
+static void alloc_tensor(Context& c, rr::Tensor& t, size_t naux, size_t n, unsigned copies) {
+  RR_REQUIRE(copies & 7u, RR_ERR_ARG, "no tensor copy requested");
+  RR_HIP(hipStreamSynchronize(c.stream));
+  if (t.f64) RR_HIP(hipFree(t.f64));
+  if (t.f32) RR_HIP(hipFree(t.f32));
+  if (t.b16) RR_HIP(hipFree(t.b16));
+  t = rr::Tensor();
+  t.naux = naux, t.n = n, t.ld = rr::roundup(std::max<size_t>(n, 1), 8);
+  const size_t cnt = std::max<size_t>(1, naux * n * t.ld);
+  if (copies & RR_COPY_FP64) RR_HIP(hipMalloc(&t.f64, cnt * sizeof(double)));
+  if (copies & RR_COPY_FP32) RR_HIP(hipMalloc(&t.f32, cnt * sizeof(float)));
+  if (copies & RR_COPY_BF16) RR_HIP(hipMalloc(&t.b16, cnt * sizeof(rr::bf16_t)));
+}
+
+int rr_tensor_upload(rr_handle h, int slot, const double* eri3, size_t n_aux, size_t n, unsigned copies) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(slot >= 0 && slot < rr::kSlots, RR_ERR_ARG, "tensor slot out of range");
+    RR_REQUIRE(eri3 || n_aux * n == 0, RR_ERR_ARG, "null tensor");
+    rr::Tensor& t = c.slot[slot];
+    alloc_tensor(c, t, n_aux, n, copies);
+    if (n_aux == 0 || n == 0) return;
+    const bool dev = rr::is_device_ptr(eri3);
+    const size_t rowbytes = n * n * sizeof(double);
+    size_t chunk = std::max<size_t>(1, std::min<size_t>(n_aux, (size_t(256) << 20) / rowbytes));
+    chunk = std::min<size_t>(chunk, 65535);
+    double* stage = dev ? nullptr : static_cast<double*>(rr::wsbuf(c, rr::WS_STAGE, chunk * rowbytes));
+    for (size_t p0 = 0; p0 < n_aux; p0 += chunk) {
+      const size_t rows = std::min(chunk, n_aux - p0);
+      const double* src = eri3 + p0 * n * n;
+      if (!dev) {
+        RR_HIP(hipMemcpyAsync(stage, src, rows * rowbytes, hipMemcpyHostToDevice, c.stream));
+        src = stage;
+      }
+      dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32), (unsigned)rows);
+      if (t.f64) hipLaunchKernelGGL(rr::k_store_rows<double>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f64 + p0 * n * t.ld);
+      if (t.f32) hipLaunchKernelGGL(rr::k_store_rows<float>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f32 + p0 * n * t.ld);
+      if (t.b16) hipLaunchKernelGGL(rr::k_store_rows<rr::bf16_t>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.b16 + p0 * n * t.ld);
+      RR_HIP(hipGetLastError());
+      if (!dev) RR_HIP(hipStreamSynchronize(c.stream));
+    }
+  });
+}
+
+int rr_tensor_generate(rr_handle h, int slot, size_t n_aux, size_t n, double sigma, double decay, uint64_t seed,
+                       unsigned copies) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(slot >= 0 && slot < rr::kSlots, RR_ERR_ARG, "tensor slot out of range");
+    RR_REQUIRE(decay > 0.0, RR_ERR_ARG, "decay length must be positive");
+    rr::Tensor& t = c.slot[slot];
+    alloc_tensor(c, t, n_aux, n, copies);
+    if (n_aux == 0 || n == 0) return;
+    const size_t total = n_aux * n * t.ld;
+    if (t.f64) hipLaunchKernelGGL(rr::k_generate<double>, dim3(rr::grid_for(total, 256, 1 << 20)), dim3(256), 0, c.stream,
+                                  n_aux, n, t.ld, sigma, decay, seed, (size_t)0, n_aux, t.f64);
+    if (t.f32) hipLaunchKernelGGL(rr::k_generate<float>, dim3(rr::grid_for(total, 256, 1 << 20)), dim3(256), 0, c.stream,
+                                  n_aux, n, t.ld, sigma, decay, seed, (size_t)0, n_aux, t.f32);
+    if (t.b16) hipLaunchKernelGGL(rr::k_generate<rr::bf16_t>, dim3(rr::grid_for(total, 256, 1 << 20)), dim3(256), 0,
+                                  c.stream, n_aux, n, t.ld, sigma, decay, seed, (size_t)0, n_aux, t.b16);
+    RR_HIP(hipGetLastError());
+  });
+}
+
+int rr_tensor_info(rr_handle h, int slot, size_t* n_aux, size_t* n, unsigned* copies) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(slot >= 0 && slot < rr::kSlots, RR_ERR_ARG, "tensor slot out of range");
+    const rr::Tensor& t = c.slot[slot];
+    if (n_aux) *n_aux = t.naux;
+    if (n) *n = t.n;
+    if (copies) *copies = (t.f64 ? RR_COPY_FP64 : 0u) | (t.f32 ? RR_COPY_FP32 : 0u) | (t.b16 ? RR_COPY_BF16 : 0u);
+  });
+}
+
+int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double* density, const double* lfac,
+                    const double* rfac, size_t r, double* j_out, double* k_out) {
+  return guarded(h, [&](Context& c) {
+    rr::check_prec(precision);
+    rr::Tensor& t = rr::get_slot(c, slot);
+    RR_REQUIRE(t.n == n, RR_ERR_DIM, "rr_ri_jk_factor: density dimension does not match the tensor");
+    double* jpart = nullptr;
+    int jsplit = 0;
+    if (j_out) {
+      RR_REQUIRE(density, RR_ERR_ARG, "rr_ri_jk_factor: J requested without a density");
+      const double* dD = rr::in_buf(c, rr::WS_P, density, n * n);
+      jsplit = rr::coulomb_real_prec(c, t, precision, dD, &jpart);
+    }
+    rr::ExchangeOut ex;
+    if (k_out && r > 0) {
+      RR_REQUIRE(lfac, RR_ERR_ARG, "rr_ri_jk_factor: K requested without a factor");
+      const double* dL = rr::in_buf(c, rr::WS_CA, lfac, n * r);
+      const double* dR = rfac ? rr::in_buf(c, rr::WS_CB, rfac, n * r) : nullptr;
+      // Lt = L^T (r x n)
+      double* lt = static_cast<double*>(rr::wsbuf(c, rr::WS_T1, 2 * r * n * sizeof(double)));
+      hipLaunchKernelGGL((rr::k_pack<double>), dim3(rr::grid_for(r * n)), dim3(256), 0, c.stream, dL, r, 1, r, r, n, n, 1.0,
+                         lt);
+      if (dR)
+        hipLaunchKernelGGL((rr::k_pack<double>), dim3(rr::grid_for(r * n)), dim3(256), 0, c.stream, dR, r, 1, r, r, n, n,
+                           1.0, lt + r * n);
+      RR_HIP(hipGetLastError());
+      ex = rr::exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n);
+    }
+    rr::OutBuf oj = rr::out_buf(c, rr::WS_OUT_J, j_out, n * n), ok = rr::out_buf(c, rr::WS_OUT_K, k_out, n * n);
+    if (k_out && r == 0) hipLaunchKernelGGL(rr::k_zero, dim3(rr::grid_for(n * n)), dim3(256), 0, c.stream, ok.dev, n * n);
+    rr::real_assemble(c, t, jpart, jsplit, (k_out && r > 0) ? &ex : nullptr, -0.5, nullptr, nullptr, oj.dev,
+                      (k_out && r > 0) ? ok.dev : nullptr);
+    rr::finish_out(c, oj);
+    rr::finish_out(c, ok);
+  });
+}
+
+int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const double* h_core, const double* x_large,
+                      size_t nx, const double* overlap, const double* density, double* fock, double* j_out,
+                      double* k_out) {
+  return guarded(h, [&](Context& c) {
+    rr::check_prec(precision);
+    rr::Tensor& t = rr::get_slot(c, slot);
+    RR_REQUIRE(t.n == nl, RR_ERR_DIM, "nonRelFockMatrix (RI): density_matrix dimension does not match nl");
+    RR_REQUIRE(density && h_core && x_large && overlap, RR_ERR_ARG, "nonRelFockMatrix (RI): null input");
+    const double* dP = rr::in_buf(c, rr::WS_P, density, nl * nl);
+    const double* dH = rr::in_buf(c, rr::WS_H, h_core, nl * nl);
+    const double* dS = rr::in_buf(c, rr::WS_S, overlap, nl * nl);
+    const double* dX = rr::in_buf(c, rr::WS_X, x_large, nl * nx);
+    double* jpart = nullptr;
+    const int jsplit = rr::coulomb_real_prec(c, t, precision, dP, &jpart);
+    // Recovery: D = P/2, C_occ = X Y_occ; c_t = sqrt(2) C_occ^T = sqrt(2) Y_occ^T X^T.
+    const double* rows = nullptr;
+    std::vector<double> occ;
+    const size_t n_occ = rr::recover_occupied(c, false, nl, nx, dS, dP, dX, 0.5, &rows, occ);
+    rr::ExchangeOut ex;
+    if (n_occ > 0) {
+      double* lt = static_cast<double*>(rr::wsbuf(c, rr::WS_COCC, n_occ * nl * sizeof(double)));
+      rr::dgemm_small(c, false, 'N', 'T', n_occ, nl, nx, rows, nx, dX, nx, lt, nl, std::sqrt(2.0));  // Y^T X^T
+      ex = rr::exchange_real_prec(c, t, precision, lt, nullptr, n_occ, nl);
+    }
+    rr::OutBuf of = rr::out_buf(c, rr::WS_OUT_F, fock, nl * nl), oj = rr::out_buf(c, rr::WS_OUT_J, j_out, nl * nl),
+               ok = rr::out_buf(c, rr::WS_OUT_K, k_out, nl * nl);
+    if (ok.dev && n_occ == 0) hipLaunchKernelGGL(rr::k_zero, dim3(rr::grid_for(nl * nl)), dim3(256), 0, c.stream, ok.dev, nl * nl);
+    rr::real_assemble(c, t, jpart, jsplit, n_occ > 0 ? &ex : nullptr, -0.5, dH, of.dev, oj.dev, n_occ > 0 ? ok.dev : nullptr);
+    rr::finish_out(c, of);
+    rr::finish_out(c, oj);
+    rr::finish_out(c, ok);
+  });
+}
+
+int rr_ri_jk_delta(rr_handle h, int slot, int precision, size_t n, size_t n_occ, const double* c_new,
+                   const double* c_old, double* dj_out, double* dk_out) {
+  return guarded(h, [&](Context& c) {
+    rr::check_prec(precision);
+    rr::Tensor& t = rr::get_slot(c, slot);
+    RR_REQUIRE(t.n == n, RR_ERR_DIM, "rr_ri_jk_delta: orbital dimension does not match the tensor");
+    RR_REQUIRE(c_new && c_old, RR_ERR_ARG, "rr_ri_jk_delta: null orbitals");
+    const double* dA = rr::in_buf(c, rr::WS_CA, c_new, n * n_occ);
+    const double* dB = rr::in_buf(c, rr::WS_CB, c_old, n * n_occ);
+    rr::OutBuf oj = rr::out_buf(c, rr::WS_OUT_J, dj_out, n * n), ok = rr::out_buf(c, rr::WS_OUT_K, dk_out, n * n);
+    rr::delta_build(c, t, precision, n, n_occ, dA, dB, oj.dev, ok.dev);
+    rr::finish_out(c, oj);
+    rr::finish_out(c, ok);
+  });
+}
+
+}  // extern "C"
