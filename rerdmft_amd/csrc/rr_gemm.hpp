@@ -36,6 +36,10 @@ struct GemmArgs {
   // written to C2 at (m - mcut, n) or (m, n - ncut) with the same strides.
   void* C2 = nullptr;
   int mcut = 0, ncut = 0;
+  // k-batched reduction: the contraction runs over kbatch blocks of K each; block b of A (B) starts
+  // sAk (sBk) logical elements after block 0. kper then counts flat k over kbatch * K.
+  int kbatch = 0;
+  long long sAk = 0, sBk = 0;
 };
 
 template <typename TO>
@@ -142,7 +146,8 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
 
   const long long kbeg = static_cast<long long>(split) * g.kper * CW;
   long long kend = kbeg + static_cast<long long>(g.kper) * CW;
-  const long long ktot = static_cast<long long>(g.K) * CW;
+  const long long kblk = static_cast<long long>(g.K) * CW;  // real elements per k-block
+  const long long ktot = kblk * (g.kbatch > 0 ? g.kbatch : 1);
   if (kend > ktot) kend = ktot;
 
   const TI* Ab = static_cast<const TI*>(g.A) + static_cast<long long>(batch) * g.sAb * CW;
@@ -177,18 +182,24 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
   for (long long k = kbeg; k < kend; k += SLAB) {
     const long long kk = k + grp * KL;
     const bool kin = kk < kend;
+    long long ka = kk, kb = kk;
+    if (g.kbatch > 0) {
+      const long long blk = kk / kblk, kl = kk - blk * kblk;
+      ka = blk * g.sAk * CW + kl;
+      kb = blk * g.sBk * CW + kl;
+    }
     frag fa[WM], fb[WN];
 #pragma unroll
     for (int i = 0; i < WM; ++i) {
       if (kin && arow[i])
-        MM::load(fa[i], arow[i] + kk);
+        MM::load(fa[i], arow[i] + ka);
       else
         MM::zero(fa[i]);
     }
 #pragma unroll
     for (int j = 0; j < WN; ++j) {
       if (kin && brow[j])
-        MM::load(fb[j], brow[j] + kk);
+        MM::load(fb[j], brow[j] + kb);
       else
         MM::zero(fb[j]);
     }
