@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--n", type=int, default=N)
     p.add_argument("--nocc", type=int, default=N_OCC)
     p.add_argument("--delta-precision", default="bf16", choices=["bf16", "f32", "f64"])
-    p.add_argument("--cpu-sample", type=int, default=240, help="aux rows timed on the CPU (0: skip)")
+    p.add_argument("--cpu-sample", type=int, default=1500, help="aux rows timed on the CPU (0: skip)")
     p.add_argument("--no-fp64-leg", action="store_true")
     return p.parse_args()
 
@@ -93,7 +93,8 @@ def main():
     dK = torch.empty_like(K)
 
     def full_build(k, prec=rr.FP64):
-        ctx.ri_jk_factor(0, d_dev[k], l_dev[k], precision=prec, j_out=J, k_out=K)
+        # D_k = L_k L_k^T: the Coulomb contraction is fused into the exchange half-transform
+        ctx.ri_jk_factor(0, None, l_dev[k], precision=prec, j_out=J, k_out=K)
         return J - 0.5 * K  # two-electron part of F (h = 0)
 
     def mixed_sequence():
@@ -121,7 +122,12 @@ def main():
         pre, f_mixed = mixed_sequence()
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t_start
-    phases = ctx.timing(enable=False, reset=True)
+    phases_p = ctx.timing(enable=False, reset=True, by_precision=True)
+    phases = {}
+    for (pn, ph), (ms_, c_) in phases_p.items():
+        phases.setdefault(ph, [0.0, 0])
+        phases[ph][0] += ms_
+        phases[ph][1] += c_
 
     # accuracy at D_20 against the FP64 build
     f_ref = full_build(N_DELTA).clone()
@@ -142,54 +148,64 @@ def main():
         fp64_sequence()
         torch.cuda.synchronize()
         t_fp64 = time.perf_counter() - t1
-        fp64_phases = ctx.timing(enable=False, reset=True)
+        fp64_phases = ctx.timing(enable=False, reset=True, by_precision=True)
         fp64_rate = builds / t_fp64
 
     rate = builds * a.steps / t_mixed
 
-    # roofline of the dominant kernel in the mixed sequence (phase times are per-launch averages
-    # from HIP events on the library stream)
-    name, (tot_ms, cnt) = max(phases.items(), key=lambda kv: kv[1][0])
-    flop_half_full = 2.0 * naux * n * n * nocc          # exchange half-transform, full build (r = n_occ)
-    flop_half_delta = 2.0 * naux * n * n * 2 * nocc     # incremental build (r = 2 n_occ)
-    flop_rank_full = 1.0 * naux * n * n * nocc          # Hermitian rank-k (lower triangle)
-    flop_rank_delta = 2.0 * naux * n * n * nocc         # full G = W_M W_dC^T
-    bytes_stream = {rr.FP64: 8, rr.FP32: 4, rr.BF16: 2}
+    # Roofline of the dominant kernel of the mixed sequence: the (phase, precision) with the largest
+    # device time, from HIP events recorded on the library stream around each launch.
+    rp = (nocc + 7) // 8 * 8
+    prec_bytes = {"f64": 8, "f32": 4, "bf16": 2}
     n_full = 2 * a.steps
     n_delta = N_DELTA * a.steps
-    if name in ("exchange_half", "exchange_rank_k"):
-        if name == "exchange_half":
-            work = n_full * flop_half_full + n_delta * flop_half_delta
-        else:
-            work = n_full * flop_rank_full + n_delta * flop_rank_delta
-        achieved = work / (tot_ms * 1e-3) / 1e12
-        # peak: time-weighted between the fp64 launches and the low-precision launches
-        peak = PEAK["bf16"] if dprec == rr.BF16 else (PEAK["f32"] if dprec == rr.FP32 else PEAK["f64"])
-        roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None}
-    else:
-        per = naux * n * n
-        work = n_full * per * 8 + n_delta * per * bytes_stream[dprec]
-        achieved = work / (tot_ms * 1e-3) / 1e9
-        roof = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK["hbm"], "unit": "GB/s",
-                "frac": achieved / PEAK["hbm"], "traffic": None}
 
-    # per-kernel rooflines of the FP64 build (for the record)
+    def algorithmic(phase, pname, launches_full, launches_delta):
+        """(work per launch, unit, bound) for a phase; full builds use rank n_occ, incremental 2 n_occ."""
+        nb = prec_bytes[pname]
+        if phase == "exchange_half":
+            per = 2.0 * naux * n * n * nocc * (2 if launches_delta else 1)
+            return per, "TFLOP/s", "mfma"
+        if phase == "exchange_rank_k":
+            return 2.0 * naux * n * n * nocc, "TFLOP/s", "mfma"
+        if phase == "coulomb":
+            return 1.0 * naux * n * n * nb, "GB/s", "hbm"
+        if phase == "gamma":
+            return 1.0 * naux * n * nocc * (2 if launches_delta else 1) * nb, "GB/s", "hbm"
+        return None, None, None
+
+    dom_key, (dom_ms, dom_cnt) = max(((k, v) for k, v in phases_p.items() if v[1] > 0), key=lambda kv: kv[1][0])
+    dom_phase_prec = dom_key[0]
+    dom_phase = dom_key[1]
+    is_delta = dom_phase_prec != "f64"  # low-precision launches are the incremental builds
+    work, unit, bound = algorithmic(dom_phase, dom_phase_prec, 0 if is_delta else n_full, n_delta if is_delta else 0)
+    avg_ms = dom_ms / dom_cnt
+    if unit == "TFLOP/s":
+        achieved = work / (avg_ms * 1e-3) / 1e12
+        peak = PEAK[dom_phase_prec]
+    else:
+        achieved = work / (avg_ms * 1e-3) / 1e9
+        peak = PEAK["hbm"]
+    roof = {"kernel": f"{dom_phase}[{dom_phase_prec}]", "bound": bound, "achieved": achieved, "peak": peak,
+            "unit": unit, "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_ms,
+            "algorithmic_per_launch": work, "launches": dom_cnt}
+
+    # per-kernel rooflines of the plain FP64 build (for the record)
     fp64_kernels = {}
     if fp64_phases:
-        nb = builds
         for k2, (ms2, c2) in fp64_phases.items():
-            if c2 == 0:
+            if c2 == 0 or k2[0] != "f64":
                 continue
             avg = ms2 / c2
-            if k2 in ("gamma", "coulomb"):
-                fp64_kernels[k2] = {"avg_ms": avg, "GB/s": naux * n * n * 8 / (avg * 1e-3) / 1e9}
-            elif k2 == "exchange_half":
-                fp64_kernels[k2] = {"avg_ms": avg, "TFLOP/s": flop_half_full / (avg * 1e-3) / 1e12}
-            elif k2 == "exchange_rank_k":
-                fp64_kernels[k2] = {"avg_ms": avg, "TFLOP/s(alg, full K)": 2 * flop_rank_full / (avg * 1e-3) / 1e12}
+            w2, u2, _ = algorithmic(k2[1], "f64", 1, 0)
+            if w2 is None:
+                fp64_kernels[k2[1]] = {"avg_ms": avg}
+            elif u2 == "TFLOP/s":
+                fp64_kernels[k2[1]] = {"avg_ms": avg, "TFLOP/s": w2 / (avg * 1e-3) / 1e12,
+                                       "frac": w2 / (avg * 1e-3) / 1e12 / PEAK["f64"]}
             else:
-                fp64_kernels[k2] = {"avg_ms": avg}
+                fp64_kernels[k2[1]] = {"avg_ms": avg, "GB/s": w2 / (avg * 1e-3) / 1e9,
+                                       "frac": w2 / (avg * 1e-3) / 1e9 / PEAK["hbm"]}
 
     cpu = None
     if a.cpu_sample > 0:
@@ -221,7 +237,7 @@ def main():
         "E_two_electron": e_ref,
         "roofline": roof,
         "fp64_build_kernels": fp64_kernels,
-        "phase_ms_mixed": {k: v[0] for k, v in phases.items()},
+        "phase_ms_mixed": {f"{k[1]}[{k[0]}]": round(v[0], 3) for k, v in phases_p.items() if v[1]},
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }

@@ -44,13 +44,14 @@ typedef struct rr_context* rr_handle;
 int rr_create(int device, rr_handle* out);
 int rr_destroy(rr_handle h);
 const char* rr_last_error(rr_handle h);
-/* Select a GEMM tile configuration for every launch (-1: per-shape autotuned choice). Results do
- * not depend on the choice. */
+/* Select the GEMM tile configuration: -1 per-shape automatic choice; 0..7 a register-direct
+ * configuration for every launch; 100..106 an LDS-staged configuration for the hot real GEMMs.
+ * Results do not depend on the choice (every element is summed in the same order). */
 int rr_set_tile_config(rr_handle h, int cfg);
-/* Per-phase device time (HIP events on the library stream) accumulated since the last reset:
- * ms[0] gamma stream, ms[1] Coulomb stream, ms[2] exchange half-transform GEMM, ms[3] exchange
- * rank-k GEMM, ms[4] assembly, ms[5] recovery; counts[i] = launches timed. enable=0 turns the
- * event timing off (default off). */
+/* Per-phase device time (HIP events on the library stream) accumulated since the last reset,
+ * indexed 8 * precision + phase (n up to 24): phase 0 Coulomb gamma contraction, 1 Coulomb J
+ * stream, 2 exchange half-transform GEMM, 3 exchange rank-k GEMM, 4 assembly, 5 recovery;
+ * counts[i] = launches timed. enable=0 turns the event timing off (default off). */
 int rr_timing(rr_handle h, int enable, int reset, double* ms, long long* counts, int n);
 
 /* ---- 3-index tensors (AO_ints/ThreeCenterIntegrals.h:42-70 layout: (n_aux, n*n), row P is the
@@ -72,7 +73,9 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
 
 /* Factor-form real J/K without recovery (the exchange loop of NonRelFockMatrixRi.cpp:74-94 with a
  * caller-supplied factor): J = sum_P tr(B_P D) B_P, K = sum_P B_P^T (L R^T) B_P with L, R n x r
- * (rfac NULL: R = L). Either of j_out / k_out may be NULL. */
+ * (rfac NULL: R = L). Either of j_out / k_out may be NULL. density NULL with j_out set means
+ * D = L R^T: the Coulomb contraction is then taken from the half-transformed factors inside the
+ * exchange GEMM, saving one pass over the tensor. */
 int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double* density, const double* lfac,
                     const double* rfac, size_t r, double* j_out, double* k_out);
 

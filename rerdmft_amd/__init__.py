@@ -134,12 +134,18 @@ class Context:
     def set_tile_config(self, cfg):
         self._call("rr_set_tile_config", ctypes.c_int(cfg))
 
-    def timing(self, enable=True, reset=False):
-        ms = (ctypes.c_double * 8)()
-        cnt = (ctypes.c_longlong * 8)()
-        self._call("rr_timing", ctypes.c_int(int(enable)), ctypes.c_int(int(reset)), ms, cnt, ctypes.c_int(8))
-        names = ["gamma", "coulomb", "exchange_half", "exchange_rank_k", "assembly", "recovery"]
-        return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
+    PHASES = ["gamma", "coulomb", "exchange_half", "exchange_rank_k", "assembly", "recovery"]
+
+    def timing(self, enable=True, reset=False, by_precision=False):
+        """Per-phase device time (ms, launches) since the last reset; by_precision splits by operand type."""
+        ms = (ctypes.c_double * 24)()
+        cnt = (ctypes.c_longlong * 24)()
+        self._call("rr_timing", ctypes.c_int(int(enable)), ctypes.c_int(int(reset)), ms, cnt, ctypes.c_int(24))
+        if by_precision:
+            return {(PRECISION_NAMES[p], k): (ms[8 * p + i], cnt[8 * p + i])
+                    for p in range(3) for i, k in enumerate(self.PHASES)}
+        return {k: (sum(ms[8 * p + i] for p in range(3)), sum(cnt[8 * p + i] for p in range(3)))
+                for i, k in enumerate(self.PHASES)}
 
     # ---- tensors -------------------------------------------------------------------------------
     def upload(self, slot, eri3, copies=COPY_FP64):
@@ -183,6 +189,7 @@ class Context:
                      k_out=None):
         """J = sum_P tr(B_P D) B_P, K = sum_P B_P^T (L R^T) B_P."""
         n = lfac.shape[0] if lfac is not None else density.shape[0]
+        # density=None: J of D = L R^T from the half-transformed factors (fused Coulomb contraction)
         r = lfac.shape[1] if lfac is not None else 0
         d, l, rr_ = _f64(density), _f64(lfac), _f64(rfac)
         j, jp = _out(j_out, (n, n), np.float64) if want_j else (None, ctypes.c_void_p(0))

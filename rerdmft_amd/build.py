@@ -21,6 +21,9 @@ FLAGS = [
     # packed fp32 VALU forms disabled on purpose (device code only; the host pass warns, harmlessly)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    # keep matrix-core accumulators in the VGPR form: with the AGPR form the allocator copied the
+    # accumulators between the two register files on every k-stage of the fp64 GEMM loops
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
@@ -34,12 +37,26 @@ def build_library(verbose=False):
     return LIB
 
 
+def build_adapter_test(verbose=False):
+    """C++ drop-in test of the adapter header (tests/cpp/test_adapter.cpp) linked to the library."""
+    out_dir = os.path.join(HERE, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "test_adapter")
+    cmd = ["g++", "-O2", "-std=c++17", os.path.join(ROOT, "tests", "cpp", "test_adapter.cpp"), "-o", exe,
+           "-L" + HERE, "-lrerdmft_gpu", "-Wl,-rpath," + HERE, "-ldl"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return exe
+
+
 def build_oracle():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
 def main():
     build_library(verbose=True)
+    build_adapter_test(verbose=True)
     build_oracle()
     print("built", LIB)
 

@@ -100,6 +100,16 @@ struct Mma<double> {
 #pragma unroll
     for (int s = 0; s < 4; ++s) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a.v[s], b.v[s], c, 0, 0, 0);
   }
+  // All tiles of a wave, step-major so consecutive instructions are independent.
+  template <int WM, int WN>
+  __device__ static void mma_tiles(acc_t (&c)[WM][WN], const frag (&a)[WM], const frag (&b)[WN]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) c[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].v[s], b[j].v[s], c[i][j], 0, 0, 0);
+  }
 };
 
 template <>
@@ -126,6 +136,15 @@ struct Mma<float> {
 #pragma unroll
     for (int s = 0; s < 8; ++s) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v[s], b.v[s], c, 0, 0, 0);
   }
+  template <int WM, int WN>
+  __device__ static void mma_tiles(acc_t (&c)[WM][WN], const frag (&a)[WM], const frag (&b)[WN]) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) c[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].v[s], b[j].v[s], c[i][j], 0, 0, 0);
+  }
 };
 
 template <>
@@ -144,6 +163,13 @@ struct Mma<bf16_t> {
   }
   __device__ static void mma(acc_t& c, const frag& a, const frag& b) {
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, c, 0, 0, 0);
+  }
+  template <int WM, int WN>
+  __device__ static void mma_tiles(acc_t (&c)[WM][WN], const frag (&a)[WM], const frag (&b)[WN]) {
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j) c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i].v, b[j].v, c[i][j], 0, 0, 0);
   }
 };
 

@@ -42,6 +42,7 @@ struct GemmArgs {
   long long sAk = 0, sBk = 0;
 };
 
+
 template <typename TO>
 __device__ inline void store_out(TO* p, double v);
 template <>
@@ -204,10 +205,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
         MM::zero(fb[j]);
     }
     if constexpr (!CPLX) {
-#pragma unroll
-      for (int i = 0; i < WM; ++i)
-#pragma unroll
-        for (int j = 0; j < WN; ++j) MM::mma(acc[i][j], fa[i], fb[j]);
+      MM::mma_tiles(acc, fa, fb);
     } else {
       // Interleaved (re, im) pairs; conj(A) is applied to the A fragment itself, then
       // Re C = a . bR and Im C = a . bI with, per pair (br, bi):
@@ -257,6 +255,176 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
           if (g.beta != 0.0) vi += g.beta * Conv<TO>::from(p[1]);
           store_out<TO>(p + 1, vi);
         }
+      }
+    }
+}
+
+}  // namespace rr
+
+namespace rr {
+
+// ------------------------------------------------------------------------------------------------
+// LDS-staged NT GEMM for the hot real contractions (exchange half-transform and rank-k update).
+// A workgroup of 4 waves (2 x 2) computes a BM x BN tile; per k-stage of BK elements the A and B
+// panels are loaded from global memory in coalesced 16-byte vectors into registers, written to a
+// double-buffered LDS ring (rows padded by 16 bytes), and consumed by the waves as matrix-core
+// fragments. The global loads of stage s+1 are in flight while stage s is computed; one barrier
+// per stage. Supports batches, split-K partials, lower-triangle tiles and the second output.
+// ------------------------------------------------------------------------------------------------
+template <typename TI, typename TO, int BM, int BN, int BK, int WGM>
+__global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
+  using MM = Mma<TI>;
+  using acc_t = typename MM::acc_t;
+  using frag = typename MM::frag;
+  constexpr int KL = MM::KL;
+  constexpr int SUB = 4 * KL;                  // k per fragment sub-step
+  static_assert(BK % SUB == 0, "BK must be a multiple of the fragment k-slab");
+  constexpr int WGN = 4 / WGM;                 // waves along N
+  constexpr int WM = BM / (16 * WGM), WN = BN / (16 * WGN);  // 16x16 tiles per wave
+  static_assert(WM * 16 * WGM == BM && WN * 16 * WGN == BN, "tile not divisible over the waves");
+  constexpr int VE = 16 / sizeof(TI);          // elements per 16-byte vector
+  constexpr int PAD = VE;                      // row padding (16 bytes)
+  constexpr int LDR = BK + PAD;                // LDS row stride (elements)
+  constexpr int VPR = BK / VE;                 // vectors per row per stage
+  constexpr int AV = (BM * VPR + 255) / 256;    // A vectors per thread per stage
+  constexpr int BV = (BN * VPR + 255) / 256;
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+  __shared__ __attribute__((aligned(16))) TI lds[2][(BM + BN) * LDR];
+
+  const int total = g.ntiles * g.nbatch * g.nsplit;
+  int w = static_cast<int>(blockIdx.x);
+  if (g.xcd_remap) w = remap_xcd(w, total);
+  const int tile = w % g.ntiles;
+  const int rest = w / g.ntiles;
+  const int batch = rest % g.nbatch;
+  const int split = rest / g.nbatch;
+  int tm, tn;
+  if (g.tri) {
+    int t = tile;
+    int a = static_cast<int>((sqrtf(8.0f * static_cast<float>(t) + 1.0f) - 1.0f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    while (a * (a + 1) / 2 > t) --a;
+    tm = a;
+    tn = t - a * (a + 1) / 2;
+  } else {
+    tm = tile % g.tiles_m;
+    tn = tile / g.tiles_m;
+  }
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, grp = lane >> 4;
+  const int wm0 = (wave % WGM) * (16 * WM), wn0 = (wave / WGM) * (16 * WN);
+  const int bm0 = tm * BM, bn0 = tn * BN;
+
+  const long long kbeg = static_cast<long long>(split) * g.kper;
+  long long kend = kbeg + g.kper;
+  if (kend > g.K) kend = g.K;
+
+  const TI* Ab = static_cast<const TI*>(g.A) + static_cast<long long>(batch) * g.sAb;
+  const TI* Bb = static_cast<const TI*>(g.B) + static_cast<long long>(batch) * g.sBb;
+
+  // per-thread global vector slots: vector v of the stage -> (row, col-vector)
+  const TI* asrc[AV];
+  int arow_l[AV], acol[AV];
+#pragma unroll
+  for (int v = 0; v < AV; ++v) {
+    int id = tid + v * 256;
+    const bool ok = id < BM * VPR;
+    if (!ok) id = 0;
+    arow_l[v] = ok ? id / VPR : -1;
+    acol[v] = (id % VPR) * VE;
+    const int m = bm0 + id / VPR;
+    asrc[v] = (ok && m < g.M) ? Ab + static_cast<long long>(m) * g.lda : nullptr;
+  }
+  const TI* bsrc[BV];
+  int brow_l[BV], bcol[BV];
+#pragma unroll
+  for (int v = 0; v < BV; ++v) {
+    int id = tid + v * 256;
+    const bool ok = id < BN * VPR;
+    if (!ok) id = 0;
+    brow_l[v] = ok ? id / VPR : -1;
+    bcol[v] = (id % VPR) * VE;
+    const int n = bn0 + id / VPR;
+    bsrc[v] = (ok && n < g.N) ? Bb + static_cast<long long>(n) * g.ldb : nullptr;
+  }
+
+  // Stage s+1 is requested from global memory while stage s is computed from LDS.
+  u4 ra[AV], rb[BV];
+  auto gload = [&](long long k0) {
+#pragma unroll
+    for (int v = 0; v < AV; ++v) {
+      const long long k = k0 + acol[v];
+      ra[v] = (asrc[v] && k < kend) ? *reinterpret_cast<const u4*>(asrc[v] + k) : u4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int v = 0; v < BV; ++v) {
+      const long long k = k0 + bcol[v];
+      rb[v] = (bsrc[v] && k < kend) ? *reinterpret_cast<const u4*>(bsrc[v] + k) : u4{0, 0, 0, 0};
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int v = 0; v < AV; ++v)
+      if (arow_l[v] >= 0) *reinterpret_cast<u4*>(&lds[buf][arow_l[v] * LDR + acol[v]]) = ra[v];
+#pragma unroll
+    for (int v = 0; v < BV; ++v)
+      if (brow_l[v] >= 0) *reinterpret_cast<u4*>(&lds[buf][(BM + brow_l[v]) * LDR + bcol[v]]) = rb[v];
+  };
+
+  acc_t acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = acc_t{};
+
+  const int nstage = static_cast<int>((kend - kbeg + BK - 1) / BK);
+  if (nstage > 0) {
+    gload(kbeg);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < nstage; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nstage) gload(kbeg + static_cast<long long>(s + 1) * BK);
+    const TI* la = &lds[buf][0];
+    const TI* lb = &lds[buf][BM * LDR];
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += SUB) {
+      frag fa[WM], fb[WN];
+#pragma unroll
+      for (int i = 0; i < WM; ++i) MM::load(fa[i], la + (wm0 + 16 * i + r) * LDR + ks + grp * KL);
+#pragma unroll
+      for (int j = 0; j < WN; ++j) MM::load(fb[j], lb + (wn0 + 16 * j + r) * LDR + ks + grp * KL);
+      MM::mma_tiles(acc, fa, fb);
+    }
+    if (s + 1 < nstage) lstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  const long long coff = static_cast<long long>(batch) * g.sCb + static_cast<long long>(split) * g.sCz;
+  TO* Cb = static_cast<TO*>(g.C) + coff;
+  TO* Cb2 = g.C2 ? static_cast<TO*>(g.C2) + coff : nullptr;
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int n = bn0 + wn0 + 16 * j + r;
+      if (n >= g.N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = bm0 + wm0 + 16 * i + MM::out_row(grp, e);
+        if (m >= g.M) continue;
+        int mm = m, nn = n;
+        TO* base = Cb;
+        if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
+        if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        TO* p = base + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
+        double v = g.alpha * static_cast<double>(acc[i][j][e]);
+        if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
+        store_out<TO>(p, v);
       }
     }
 }

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rr_gemm.hpp"
@@ -46,6 +47,10 @@ struct Context {
   Tensor slot[kSlots];
   std::map<int, std::pair<void*, size_t>> ws;
   int tile_override = -1;
+  int lds_override = -1;  // -1 auto, -2 generic kernel only, >= 0 forced LDS variant
+  bool autotune = true;
+  int last_ntiles = 0;  // tiles per batch of the most recent GEMM launch
+  std::map<std::vector<long long>, std::pair<int, int>> tuned;  // shape key -> (generic cfg, LDS variant)
   float last_ms[8] = {0};
 };
 
@@ -374,7 +379,8 @@ __global__ void k_real_asm(RealAsm a) {
     double k = 0.0;
     if (a.kpart) {
       if (a.ktile > 0) {
-        const bool lower = (mu / a.ktile) >= (nu / a.ktile);
+        // lower-triangle tiling: every element with mu >= nu was computed, for any square tile size
+        const bool lower = mu >= nu;
         const size_t src = lower ? mu * a.n + nu : nu * a.n + mu;
         for (int s = 0; s < a.ksplit; ++s) k += a.kpart[s * a.kstride + src];
       } else if (a.ksym) {
@@ -413,7 +419,7 @@ __global__ void k_cplx_asm(CplxAsm a) {
     const size_t r = idx / a.nd, c = idx % a.nd;
     double kr = 0.0, ki = 0.0;
     if (a.kpart) {
-      const bool lower = (r / a.ktile) >= (c / a.ktile);
+      const bool lower = r >= c;  // Hermitian lower-triangle tiling (any square tile size)
       const size_t src = lower ? r * a.nd + c : c * a.nd + r;
       for (int s = 0; s < a.ksplit; ++s) {
         kr += a.kpart[2 * (s * a.kstride + src)];
@@ -480,12 +486,47 @@ struct GemmSpec {
   int out = P64;       // output type
   bool cplx = false;
   int cfg = 1;
+  int lds = -1;        // >= 0: LDS-staged kernel variant (real only), see kLds
 };
+
+// LDS-staged kernel variants: (BM, BN, waves along M)
+struct LdsCfg {
+  int bm, bn, wgm;
+};
+static const LdsCfg kLds[] = {{128, 128, 2}, {128, 208, 4}, {256, 128, 2}, {64, 208, 4},
+                              {64, 224, 2},  {256, 208, 4}, {128, 224, 2}};
+constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
+
+template <typename TI, typename TO>
+void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
+  constexpr int SUB = 4 * Mma<TI>::KL;
+  constexpr bool B16 = std::is_same<TI, bf16_t>::value;
+  constexpr int BK2 = 2 * SUB;  // two fragment sub-steps per stage
+  switch (v) {
+    case 0: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 128, 128, (B16 ? BK2 : SUB), 2>), dim3(grid), dim3(256), 0, s, g); break;
+    case 1: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 128, 208, SUB, 4>), dim3(grid), dim3(256), 0, s, g); break;
+    case 3: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 64, 208, SUB, 4>), dim3(grid), dim3(256), 0, s, g); break;
+    case 4: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 64, 224, (B16 ? BK2 : SUB), 2>), dim3(grid), dim3(256), 0, s, g); break;
+    default:
+      // the wide variants exceed the fp64 register budget and are only built for fp32 / bf16
+      if constexpr (!std::is_same<TI, double>::value) {
+        switch (v) {
+          case 2: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 256, 128, (B16 ? BK2 : SUB), 2>), dim3(grid), dim3(256), 0, s, g); return;
+          case 5: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 256, 208, SUB, 4>), dim3(grid), dim3(256), 0, s, g); return;
+          case 6: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 128, 224, (B16 ? BK2 : SUB), 2>), dim3(grid), dim3(256), 0, s, g); return;
+          default: break;
+        }
+      }
+      throw Error{RR_ERR_ARG, "LDS tile configuration not available for this precision"};
+  }
+}
 
 void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
   RR_REQUIRE(sp.cfg >= 0 && sp.cfg < kNumTiles, RR_ERR_ARG, "bad tile configuration");
+  const bool use_lds = sp.lds >= 0 && !sp.cplx && g.kbatch == 0;
+  RR_REQUIRE(!use_lds || sp.lds < kNumLds, RR_ERR_ARG, "bad LDS tile configuration");
   const TileCfg t = kTiles[sp.cfg];
-  const int BM = 32 * t.wm, BN = 32 * t.wn;
+  const int BM = use_lds ? kLds[sp.lds].bm : 32 * t.wm, BN = use_lds ? kLds[sp.lds].bn : 32 * t.wn;
   g.tiles_m = (g.M + BM - 1) / BM;
   g.tiles_n = (g.N + BN - 1) / BN;
   if (g.tri) {
@@ -499,6 +540,17 @@ void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
   if (total == 0 || g.M == 0 || g.N == 0) return;
   RR_REQUIRE(total < (1ll << 31), RR_ERR_ARG, "GEMM grid too large");
   const int grid = (int)total;
+  c.last_ntiles = g.ntiles;
+  if (use_lds) {
+    if (sp.prec == P64 && sp.out == P64) launch_lds<double, double>(sp.lds, g, grid, c.stream);
+    else if (sp.prec == P32 && sp.out == P32) launch_lds<float, float>(sp.lds, g, grid, c.stream);
+    else if (sp.prec == P32 && sp.out == P64) launch_lds<float, double>(sp.lds, g, grid, c.stream);
+    else if (sp.prec == P16 && sp.out == P16) launch_lds<bf16_t, bf16_t>(sp.lds, g, grid, c.stream);
+    else if (sp.prec == P16 && sp.out == P64) launch_lds<bf16_t, double>(sp.lds, g, grid, c.stream);
+    else throw Error{RR_ERR_ARG, "unsupported GEMM precision combination"};
+    RR_HIP(hipGetLastError());
+    return;
+  }
   const bool cp = sp.cplx;
 #define RR_DISPATCH(TI, TO)                                              \
   do {                                                                   \
@@ -518,6 +570,72 @@ void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
   RR_HIP(hipGetLastError());
 }
 
+// Tile autotuner: for a large GEMM seen for the first time, every candidate configuration is run
+// once on the real operands and timed with HIP events; the fastest is cached per shape. All
+// candidates sum every output element in the same order, so the choice never changes a result.
+void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
+  if (c.tile_override >= 0 || c.lds_override >= 0 || c.lds_override == -2) {
+    if (c.tile_override >= 0) {
+      const TileCfg& t = kTiles[c.tile_override];
+      if (!g.tri || t.wm == t.wn) sp.cfg = c.tile_override;
+      sp.lds = -1;
+    }
+    if (c.lds_override >= 0 && !sp.cplx && g.kbatch == 0) {
+      const LdsCfg& v = kLds[c.lds_override];
+      const bool fp64_wide = sp.prec == P64 && (c.lds_override == 2 || c.lds_override == 5 || c.lds_override == 6);
+      if ((!g.tri || v.bm == v.bn) && !fp64_wide) sp.lds = c.lds_override;
+    }
+    if (c.lds_override == -2) sp.lds = -1;
+    run_gemm(c, sp, g);
+    return;
+  }
+  const double flops = 2.0 * g.M * g.N * (double)g.K * std::max(1, g.nbatch) * std::max(1, g.kbatch) * (sp.cplx ? 4 : 1);
+  if (!c.autotune || flops < 2e10 || g.beta != 0.0) {
+    run_gemm(c, sp, g);
+    return;
+  }
+  std::vector<long long> key = {sp.prec, sp.out, sp.cplx, g.M, g.N, g.K, g.nbatch, g.nsplit, g.tri, g.kbatch, g.ncut, g.mcut};
+  auto it = c.tuned.find(key);
+  if (it != c.tuned.end()) {
+    sp.cfg = it->second.first, sp.lds = it->second.second;
+    run_gemm(c, sp, g);
+    return;
+  }
+  std::vector<std::pair<int, int>> cand;
+  for (int cfg : {1, 2, 5}) {
+    const TileCfg t = kTiles[cfg];
+    if (g.tri && t.wm != t.wn) continue;
+    cand.push_back({cfg, -1});
+  }
+  if (!sp.cplx && g.kbatch == 0)
+    for (int v = 0; v < kNumLds; ++v) {
+      if (g.tri && kLds[v].bm != kLds[v].bn) continue;
+      if (sp.prec == P64 && (v == 2 || v == 5 || v == 6)) continue;  // these exceed the fp64 register budget
+      cand.push_back({1, v});
+    }
+  hipEvent_t e0, e1;
+  RR_HIP(hipEventCreate(&e0));
+  RR_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  std::pair<int, int> pick = {sp.cfg, sp.lds};
+  for (auto& cd : cand) {
+    GemmSpec t = sp;
+    t.cfg = cd.first, t.lds = cd.second;
+    RR_HIP(hipEventRecord(e0, c.stream));
+    run_gemm(c, t, g);
+    RR_HIP(hipEventRecord(e1, c.stream));
+    RR_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    RR_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) best = ms, pick = cd;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.tuned[key] = pick;
+  sp.cfg = pick.first, sp.lds = pick.second;
+  run_gemm(c, sp, g);  // leaves the output of the chosen configuration (identical to every other)
+}
+
 size_t prec_size(int p) { return p == P64 ? 8 : (p == P32 ? 4 : 2); }
 
 const void* tensor_data(const Tensor& t, int prec) {
@@ -530,6 +648,17 @@ const void* tensor_data(const Tensor& t, int prec) {
 // Small FP64 products on device (recovery, embeddings): C = op(A) op(B), any op in {N, T, C}.
 // Operands are packed into k-contiguous zero-padded buffers so the NT engine applies.
 // ================================================================================================
+// C[r][c] (row stride ldc) = alpha * sum_s part[s][r][c] + beta * C[r][c], part rows of width w.
+__global__ void k_reduce_scale(const double* part, int nsplit, size_t stride, size_t rows, size_t w, size_t ldc,
+                               double alpha, double beta, double* C) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < rows * w; i += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += part[z * stride + i];
+    double* out = C + (i / w) * ldc + (i % w);
+    *out = alpha * s + (beta != 0.0 ? beta * *out : 0.0);
+  }
+}
+
 __global__ void k_pack_op(const double* in, size_t ld_in, char op, int cplx, size_t R, size_t K, size_t ld_out,
                           double* out) {
   // out row r (length ld_out) = op(in) row r, i.e. for op N: in[r][k]; for T/C: in[k][r].
@@ -570,7 +699,24 @@ void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, 
   g.M = (int)m, g.N = (int)n, g.K = (int)kp, g.alpha = alpha, g.beta = beta;
   g.conj = (cplx && opb == 'C') ? 2 : 0;
   GemmSpec sp;
-  sp.prec = P64, sp.out = P64, sp.cplx = cplx, sp.cfg = (m >= 256 && n >= 256) ? 2 : 1;
+  sp.prec = P64, sp.out = P64, sp.cplx = cplx, sp.cfg = (m >= 512 && n >= 512) ? 2 : (m >= 128 && n >= 128 ? 1 : 0);
+  // Split K when the output alone gives too few workgroups (e.g. 200 x 200 x 1000 products).
+  const int bm = 32 * kTiles[sp.cfg].wm;
+  const size_t tiles = ((m + bm - 1) / bm) * ((n + bm - 1) / bm);
+  const size_t want = std::max<size_t>(1, std::min<size_t>(64, 512 / std::max<size_t>(1, tiles)));
+  const size_t ns = std::min<size_t>(want, std::max<size_t>(1, kp / 64));
+  if (ns > 1) {
+    const size_t kper = roundup((kp + ns - 1) / ns, 16);
+    const int nsplit = (int)((kp + kper - 1) / kper);
+    double* part = static_cast<double*>(wsbuf(c, WS_TMP9 + 50, (size_t)nsplit * m * n * cw * sizeof(double)));
+    GemmArgs p = g;
+    p.C = part, p.ldcm = n, p.sCz = m * n, p.kper = (int)kper, p.nsplit = nsplit, p.alpha = 1.0, p.beta = 0.0;
+    run_gemm(c, sp, p);
+    hipLaunchKernelGGL(k_reduce_scale, dim3(grid_for(m * n * cw)), dim3(256), 0, c.stream, part, nsplit, m * n * cw, m,
+                       n * cw, ldc * cw, alpha, beta, C);
+    RR_HIP(hipGetLastError());
+    return;
+  }
   run_gemm(c, sp, g);
 }
 
@@ -610,9 +756,10 @@ struct PhaseTimer {
   int phase;
   hipEvent_t a = nullptr, b = nullptr;
   static bool enabled;
-  static double ms[8];
-  static long long cnt[8];
-  PhaseTimer(Context& cc, int ph) : c(cc), phase(ph) {
+  static int cur_prec;  // precision of the build in progress (set by the entry points)
+  static double ms[24];
+  static long long cnt[24];
+  PhaseTimer(Context& cc, int ph) : c(cc), phase(ph + 8 * cur_prec) {
     if (!enabled) return;
     RR_HIP(hipEventCreate(&a));
     RR_HIP(hipEventCreate(&b));
@@ -632,8 +779,14 @@ struct PhaseTimer {
   }
 };
 bool PhaseTimer::enabled = false;
-double PhaseTimer::ms[8] = {0};
-long long PhaseTimer::cnt[8] = {0};
+int PhaseTimer::cur_prec = 0;
+double PhaseTimer::ms[24] = {0};
+long long PhaseTimer::cnt[24] = {0};
+struct PrecScope {
+  int saved;
+  explicit PrecScope(int p) : saved(PhaseTimer::cur_prec) { PhaseTimer::cur_prec = (p >= 0 && p < 3) ? p : 0; }
+  ~PrecScope() { PhaseTimer::cur_prec = saved; }
+};
 enum { PH_GAMMA = 0, PH_COUL = 1, PH_GEMM1 = 2, PH_GEMM2 = 3, PH_ASM = 4, PH_RECOVER = 5 };
 
 // ================================================================================================
@@ -700,16 +853,28 @@ int coulomb_stream(Context& c, const Tensor& t, const double* gamma, double* jpa
 }
 
 int gemm1_cfg(Context& c, size_t ncols) {
-  if (c.tile_override >= 0) return c.tile_override;
+  (void)c;
   if (ncols <= 32) return 0;   // 32 x 32
   if (ncols <= 64) return 1;   // 64 x 64
   if (ncols <= 128) return 4;  // 128 x 64 -> (4,2): BM=128, BN=64
   return 5;                    // 64 x 224
 }
 int gemm2_cfg(Context& c, size_t n) {
-  if (c.tile_override >= 0 && kTiles[c.tile_override].wm == kTiles[c.tile_override].wn) return c.tile_override;
+  (void)c;
   if (n <= 64) return 1;
   return 2;  // 128 x 128
+}
+
+// LDS-staged variants for the hot real GEMMs (-1: generic kernel). c.lds_override >= 0 forces one.
+int gemm1_lds(Context& c, size_t m, size_t ncols) {
+  (void)c;
+  if (m < 64 || ncols < 64) return -1;
+  return ncols <= 128 ? 0 : 1;
+}
+int gemm2_lds(Context& c, size_t n) {
+  (void)c;
+  if (n < 128) return -1;
+  return 0;
 }
 
 // Number of k-splits of the rank-k exchange GEMM: a function of the problem shape only, so that
@@ -732,9 +897,59 @@ struct ExchangeOut {
 
 // Real exchange in factor form on one tensor: W = Bt_P Lt^T (and W2 = Bt_P Rt^T), then
 // K partials = W W^T (tri) or W W2^T (full). Lt/Rt: device fp64, r x ld_l.
+// Coulomb contraction from the half-transformed factors: with W[nu][(P,i)] = (B_P^T L)[nu][i],
+//   gamma_P = sum_(nu,i) W[nu][(P,i)] dfac[nu][i] (+ the same over W2 with dfac columns [rp, 2rp)).
+// Pass 1: each thread owns 8 consecutive flat k = (P,i) positions (one aux row P, rp % 8 == 0) and
+// sums over a slice of nu rows; pass 2 reduces the slices and the rp/8 groups of each P in order.
+template <typename TW>
+__global__ __launch_bounds__(256) void k_gamma_w_partial(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+                                                         const double* dfac, size_t lddf, size_t rows, double* part) {
+  const size_t g8 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;  // group of 8 k
+  const size_t k0 = g8 * 8;
+  if (k0 >= kdim) return;
+  const size_t i0 = k0 % rp;
+  const size_t nu0 = blockIdx.y * rows, nu1 = std::min(n, nu0 + rows);
+  double acc = 0.0;
+  for (size_t nu = nu0; nu < nu1; ++nu) {
+    const TW* w = W + nu * kdim + k0;
+    const double* df = dfac + nu * lddf + i0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc += Conv<TW>::from(w[q]) * df[q];
+    if (W2) {
+      const TW* w2 = W2 + nu * kdim + k0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc += Conv<TW>::from(w2[q]) * df[rp + q];
+    }
+  }
+  part[blockIdx.y * (kdim / 8) + g8] = acc;
+}
+__global__ void k_gamma_w_reduce(const double* part, int nsplit, size_t kdim, size_t rp, size_t naux, double* gamma) {
+  const size_t gpp = rp / 8, ng = kdim / 8;
+  for (size_t P = blockIdx.x * (size_t)blockDim.x + threadIdx.x; P < naux; P += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z)
+      for (size_t q = 0; q < gpp; ++q) s += part[z * ng + P * gpp + q];
+    gamma[P] = s;
+  }
+}
+
+// dfac (n x ncols) for the Coulomb contraction of a factor-form density: column block b of the
+// half-transformed W is weighted by rows of fac[b] (r x ld_l, i.e. factor^T), scaled by scale[b].
+__global__ void k_build_dfac(const double* f0, const double* f1, size_t r, size_t ld_l, size_t n, size_t rp, int nblk,
+                             double s0, double s1, double* out) {
+  const size_t cols = nblk * rp;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n * cols; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t nu = idx / cols, col = idx % cols, b = col / rp, i = col % rp;
+    double v = 0.0;
+    const double* f = b == 0 ? f0 : f1;
+    if (i < r && f) v = (b == 0 ? s0 : s1) * f[i * ld_l + nu];
+    out[idx] = v;
+  }
+}
+
 template <typename TI>
 ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
-                          size_t ld_l) {
+                          size_t ld_l, const double* dfac = nullptr, double* gamma_out = nullptr) {
   ExchangeOut out;
   const size_t n = t.n, ld = t.ld, naux = t.naux;
   const size_t rp = roundup(std::max<size_t>(r, 1), 8);
@@ -760,7 +975,21 @@ ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* d
     if (dRt) g.C2 = W2, g.ncut = (int)rp;
     GemmSpec sp;
     sp.prec = prec, sp.out = prec, sp.cfg = gemm1_cfg(c, nfac * rp);
-    run_gemm(c, sp, g);
+    sp.lds = gemm1_lds(c, n, nfac * rp);
+    gemm_auto(c, sp, g);
+  }
+  if (gamma_out) {
+    PhaseTimer pt(c, PH_GAMMA);
+    const size_t ng = kdim / 8;
+    const size_t gx = (ng + 255) / 256;
+    size_t rows = std::max<size_t>(8, (n * gx + 4095) / 4096);  // ~4096 workgroups
+    const int nsplit = (int)((n + rows - 1) / rows);
+    double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * ng * sizeof(double)));
+    hipLaunchKernelGGL((k_gamma_w_partial<TI>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W,
+                       (nfac == 2 ? W2 : nullptr), kdim, rp, n, dfac, nfac * rp, rows, part);
+    hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux,
+                       gamma_out);
+    RR_HIP(hipGetLastError());
   }
   const int slab = prec == P64 ? 16 : 32;
   const int ns = gemm2_splits(n, kdim, slab);
@@ -777,8 +1006,9 @@ ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* d
     g.tri = dRt ? 0 : 1;
     GemmSpec sp;
     sp.prec = prec, sp.out = P64, sp.cfg = gemm2_cfg(c, n);
-    run_gemm(c, sp, g);
-    out.ktile = dRt ? 0 : 32 * kTiles[sp.cfg].wm;
+    sp.lds = gemm2_lds(c, n);
+    gemm_auto(c, sp, g);
+    out.ktile = dRt ? 0 : 1;
   }
   out.kpart = kpart, out.ksplit = nsplit, out.kstride = n * n;
   return out;
@@ -803,10 +1033,17 @@ int coulomb_real_prec(Context& c, const Tensor& t, int prec, const double* dD, d
 }
 
 ExchangeOut exchange_real_prec(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
-                               size_t ld_l) {
-  if (prec == P64) return exchange_real<double>(c, t, prec, dLt, dRt, r, ld_l);
-  if (prec == P32) return exchange_real<float>(c, t, prec, dLt, dRt, r, ld_l);
-  return exchange_real<bf16_t>(c, t, prec, dLt, dRt, r, ld_l);
+                               size_t ld_l, const double* dfac = nullptr, double* gamma_out = nullptr) {
+  if (prec == P64) return exchange_real<double>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out);
+  if (prec == P32) return exchange_real<float>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out);
+  return exchange_real<bf16_t>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out);
+}
+
+// Coulomb stream only (gamma given): returns split partials of Jt.
+int coulomb_from_gamma(Context& c, const Tensor& t, int prec, const double* gamma, double** jpart) {
+  if (prec == P64) return coulomb_stream<double>(c, t, gamma, nullptr, jpart);
+  if (prec == P32) return coulomb_stream<float>(c, t, gamma, nullptr, jpart);
+  return coulomb_stream<bf16_t>(c, t, gamma, nullptr, jpart);
 }
 
 void real_assemble(Context& c, const Tensor& t, double* jpart, int jsplit, const ExchangeOut* ex, double kfac,
@@ -814,9 +1051,24 @@ void real_assemble(Context& c, const Tensor& t, double* jpart, int jsplit, const
   PhaseTimer pt(c, PH_ASM);
   RealAsm a;
   a.n = t.n, a.ld = t.ld;
+  // Reduce the split partials first with coalesced reads, so that the assembly's transposed reads
+  // touch one matrix instead of every partial.
+  if (jpart && jsplit > 1) {
+    double* jr = static_cast<double*>(wsbuf(c, WS_TMP8 + 400, t.n * t.ld * sizeof(double)));
+    hipLaunchKernelGGL(k_reduce_parts, dim3(grid_for(t.n * t.ld)), dim3(256), 0, c.stream, jpart, jsplit, t.n * t.ld,
+                       t.n * t.ld, jr);
+    jpart = jr, jsplit = 1;
+  }
   a.jpart = jpart, a.jsplit = jsplit, a.jstride = t.n * t.ld;
-  if (ex && ex->kpart)
+  if (ex && ex->kpart) {
     a.kpart = ex->kpart, a.ksplit = ex->ksplit, a.kstride = ex->kstride, a.ktile = ex->ktile, a.ksym = ex->ksym;
+    if (ex->ksplit > 1) {
+      double* kr = static_cast<double*>(wsbuf(c, WS_TMP8 + 401, t.n * t.n * sizeof(double)));
+      hipLaunchKernelGGL(k_reduce_parts, dim3(grid_for(t.n * t.n)), dim3(256), 0, c.stream, ex->kpart, ex->ksplit,
+                         ex->kstride, t.n * t.n, kr);
+      a.kpart = kr, a.ksplit = 1;
+    }
+  }
   a.kfac = kfac, a.h = dh, a.F = dF, a.J = dJ, a.K = dK;
   hipLaunchKernelGGL(k_real_asm, dim3(grid_for(t.n * t.n)), dim3(256), 0, c.stream, a);
   RR_HIP(hipGetLastError());
@@ -957,25 +1209,29 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * no)), dim3(256), 0, c.stream, n * no, 1.0, cp, -1.0, dB, dc);   // dC
   hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * no)), dim3(256), 0, c.stream, n * no, 1.0, dB, 0.5, dc, m);     // M
   RR_HIP(hipGetLastError());
+  // Factors sqrt2*M and sqrt2*dC as rows (Lt, Rt). The Coulomb contraction is fused into the
+  // half-transform: gamma_P(dD) = sum_(nu,i) W_M[nu][(P,i)] sqrt2 dC[nu][i] + W_dC[nu][(P,i)] sqrt2 M[nu][i].
+  double* lt = static_cast<double*>(wsbuf(c, WS_T1 + 200, 2 * no * n * sizeof(double)));
+  hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(no * n)), dim3(256), 0, c.stream, m, no, 1, no, no, n, n,
+                     std::sqrt(2.0), lt);
+  hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(no * n)), dim3(256), 0, c.stream, dc, no, 1, no, no, n, n,
+                     std::sqrt(2.0), lt + no * n);
+  RR_HIP(hipGetLastError());
+  const size_t rp = roundup(no, 8);
+  double* dfac = nullptr;
+  double* gamma = nullptr;
+  if (dJ) {
+    dfac = static_cast<double*>(wsbuf(c, WS_DENS2, n * 2 * rp * sizeof(double)));
+    hipLaunchKernelGGL(k_build_dfac, dim3(grid_for(n * 2 * rp)), dim3(256), 0, c.stream, lt + no * n, lt, no, n, n, rp, 2,
+                       1.0, 1.0, dfac);
+    RR_HIP(hipGetLastError());
+    gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
+  }
+  ExchangeOut ex = exchange_real_prec(c, t, prec, lt, lt + no * n, no, n, dfac, gamma);
+  ex.ksym = 1;
   double* jpart = nullptr;
   int jsplit = 0;
-  if (dJ) {
-    double* dd = static_cast<double*>(wsbuf(c, WS_DENS2, n * n * sizeof(double)));
-    dgemm_small(c, false, 'N', 'T', n, n, no, m, no, dc, no, dd, n, 2.0);             // 2 M dC^T
-    dgemm_small(c, false, 'N', 'T', n, n, no, dc, no, m, no, dd, n, 2.0, 1.0);        // + 2 dC M^T
-    jsplit = coulomb_real_prec(c, t, prec, dd, &jpart);
-  }
-  ExchangeOut ex;
-  if (dK) {
-    double* lt = static_cast<double*>(wsbuf(c, WS_T1 + 200, 2 * no * n * sizeof(double)));
-    hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(no * n)), dim3(256), 0, c.stream, m, no, 1, no, no, n, n,
-                       std::sqrt(2.0), lt);
-    hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(no * n)), dim3(256), 0, c.stream, dc, no, 1, no, no, n, n,
-                       std::sqrt(2.0), lt + no * n);
-    RR_HIP(hipGetLastError());
-    ex = exchange_real_prec(c, t, prec, lt, lt + no * n, no, n);
-    ex.ksym = 1;
-  }
+  if (dJ) jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart);
   real_assemble(c, t, jpart, jsplit, dK ? &ex : nullptr, 0.0, nullptr, nullptr, dJ, dK);
 }
 
@@ -1039,7 +1295,8 @@ ExchangeOut exchange_flavors(Context& c, int prec, const std::vector<Flavor>& fl
     g.M = (int)t.n, g.N = (int)(4 * rp), g.K = (int)t.ld, g.nbatch = (int)naux;
     GemmSpec sp;
     sp.prec = prec, sp.out = prec, sp.cfg = gemm1_cfg(c, 4 * rp);
-    run_gemm(c, sp, g);
+    sp.lds = gemm1_lds(c, t.n, 4 * rp);
+    gemm_auto(c, sp, g);
   }
   const int slab = prec == P64 ? 16 : 32;
   const int ns = gemm2_splits(n_stack, 2 * kdim, slab);
@@ -1055,8 +1312,8 @@ ExchangeOut exchange_flavors(Context& c, int prec, const std::vector<Flavor>& fl
     g.tri = 1, g.conj = 1;
     GemmSpec sp;
     sp.prec = prec, sp.out = P64, sp.cplx = true, sp.cfg = gemm2_cfg(c, n_stack);
-    run_gemm(c, sp, g);
-    out.ktile = 32 * kTiles[sp.cfg].wm;
+    gemm_auto(c, sp, g);
+    out.ktile = 1;
   }
   out.kpart = kpart, out.ksplit = nsplit, out.kstride = n_stack * n_stack;
   return out;
@@ -1194,7 +1451,7 @@ void mo_real_slot(Context& c, const Tensor& t, int prec, const double* dC, size_
       g.M = (int)nmo, g.N = (int)n, g.K = (int)ld, g.nbatch = (int)rows;
       GemmSpec sp;
       sp.prec = op, sp.out = op, sp.cfg = 2;
-      run_gemm(c, sp, g);
+      gemm_auto(c, sp, g);
     }
     {
       PhaseTimer pt(c, PH_GEMM2);
@@ -1205,7 +1462,7 @@ void mo_real_slot(Context& c, const Tensor& t, int prec, const double* dC, size_
       g.M = (int)nmo, g.N = (int)nmo, g.K = (int)ld, g.nbatch = (int)rows;
       GemmSpec sp;
       sp.prec = op, sp.out = oo, sp.cfg = 2;
-      run_gemm(c, sp, g);
+      gemm_auto(c, sp, g);
     }
   }
 }
@@ -1291,7 +1548,7 @@ void mo_cplx_slot(Context& c, const std::vector<Flavor>& fl, int prec, const dou
       g.M = (int)(2 * nt), g.N = (int)t.n, g.K = (int)t.ld, g.nbatch = (int)rows;
       GemmSpec sp;
       sp.prec = op, sp.out = op, sp.cfg = 2;
-      run_gemm(c, sp, g);
+      gemm_auto(c, sp, g);
     }
     PhaseTimer pt(c, PH_GEMM2);
     GemmArgs g;
@@ -1301,7 +1558,7 @@ void mo_cplx_slot(Context& c, const std::vector<Flavor>& fl, int prec, const dou
     g.M = (int)nt, g.N = (int)(2 * nt), g.K = (int)ldk, g.nbatch = (int)rows;
     GemmSpec sp;
     sp.prec = op, sp.out = oo, sp.cfg = 2;
-    run_gemm(c, sp, g);
+    gemm_auto(c, sp, g);
   }
 }
 
@@ -1359,7 +1616,7 @@ void rotate_cplx(Context& c, int prec, size_t n, size_t naux, const double* db, 
       g.M = (int)n, g.N = (int)n, g.K = (int)ld, g.nbatch = (int)rows;
       GemmSpec sp;
       sp.prec = op, sp.out = op, sp.cplx = true, sp.cfg = 2;
-      run_gemm(c, sp, g);
+      gemm_auto(c, sp, g);
     }
     PhaseTimer pt(c, PH_GEMM2);
     GemmArgs g;
@@ -1369,7 +1626,7 @@ void rotate_cplx(Context& c, int prec, size_t n, size_t naux, const double* db, 
     g.M = (int)n, g.N = (int)n, g.K = (int)ld, g.nbatch = (int)rows, g.conj = 1;
     GemmSpec sp;
     sp.prec = op, sp.out = P64, sp.cplx = true, sp.cfg = 2;
-    run_gemm(c, sp, g);
+    gemm_auto(c, sp, g);
   }
 }
 
@@ -1402,7 +1659,7 @@ void rotate_real(Context& c, int prec, size_t n, size_t naux, const double* dx, 
       g.M = (int)n, g.N = (int)n, g.K = (int)ld, g.nbatch = (int)rows;
       GemmSpec sp;
       sp.prec = op, sp.out = op, sp.cfg = 2;
-      run_gemm(c, sp, g);
+      gemm_auto(c, sp, g);
     }
     PhaseTimer pt(c, PH_GEMM2);
     GemmArgs g;
@@ -1412,7 +1669,7 @@ void rotate_real(Context& c, int prec, size_t n, size_t naux, const double* dx, 
     g.M = (int)n, g.N = (int)n, g.K = (int)ld, g.nbatch = (int)rows;
     GemmSpec sp;
     sp.prec = op, sp.out = P64, sp.cfg = 2;
-    run_gemm(c, sp, g);
+    gemm_auto(c, sp, g);
   }
 }
 
@@ -1468,7 +1725,7 @@ void cholesky_exchange(Context& c, const Tensor& t, int prec, const double* dPop
     g.M = (int)n, g.N = (int)prow, g.K = (int)ld, g.nbatch = (int)nchol;
     GemmSpec sp;
     sp.prec = op, sp.out = op, sp.cfg = 2;
-    run_gemm(c, sp, g);
+    gemm_auto(c, sp, g);
   }
   PhaseTimer pt(c, PH_GEMM2);
   GemmArgs g;
@@ -1478,7 +1735,7 @@ void cholesky_exchange(Context& c, const Tensor& t, int prec, const double* dPop
   g.M = (int)n, g.N = (int)(cplx_rows ? 2 * n : n), g.K = (int)ld, g.kbatch = (int)nchol;
   GemmSpec sp;
   sp.prec = op, sp.out = P64, sp.cfg = 2;
-  run_gemm(c, sp, g);
+  gemm_auto(c, sp, g);
 }
 
 void cholesky_exchange_prec(Context& c, const Tensor& t, int prec, const double* dPop, size_t prow, double* kout, size_t ldk,
@@ -1597,22 +1854,27 @@ const char* rr_last_error(rr_handle h) { return h ? h->ctx.err.c_str() : "null h
 
 int rr_set_tile_config(rr_handle h, int cfg) {
   if (!h) return RR_ERR_ARG;
-  if (cfg < -1 || cfg >= rr::kNumTiles) {
+  if (cfg == -1) {
+    h->ctx.tile_override = -1, h->ctx.lds_override = -1;
+  } else if (cfg >= 0 && cfg < rr::kNumTiles) {
+    h->ctx.tile_override = cfg, h->ctx.lds_override = -2;
+  } else if (cfg >= 100 && cfg < 100 + rr::kNumLds) {
+    h->ctx.tile_override = -1, h->ctx.lds_override = cfg - 100;
+  } else {
     h->ctx.err = "tile configuration out of range";
     return RR_ERR_ARG;
   }
-  h->ctx.tile_override = cfg;
   return RR_OK;
 }
 
 int rr_timing(rr_handle h, int enable, int reset, double* ms, long long* counts, int n) {
   if (!h) return RR_ERR_ARG;
-  for (int i = 0; i < n && i < 8; ++i) {
+  for (int i = 0; i < n && i < 24; ++i) {
     if (ms) ms[i] = rr::PhaseTimer::ms[i];
     if (counts) counts[i] = rr::PhaseTimer::cnt[i];
   }
   if (reset)
-    for (int i = 0; i < 8; ++i) rr::PhaseTimer::ms[i] = 0.0, rr::PhaseTimer::cnt[i] = 0;
+    for (int i = 0; i < 24; ++i) rr::PhaseTimer::ms[i] = 0.0, rr::PhaseTimer::cnt[i] = 0;
   rr::PhaseTimer::enabled = enable != 0;
   return RR_OK;
 }
@@ -1705,12 +1967,15 @@ int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double
                     const double* rfac, size_t r, double* j_out, double* k_out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == n, RR_ERR_DIM, "rr_ri_jk_factor: density dimension does not match the tensor");
     double* jpart = nullptr;
     int jsplit = 0;
-    if (j_out) {
-      RR_REQUIRE(density, RR_ERR_ARG, "rr_ri_jk_factor: J requested without a density");
+    const bool fused_j = j_out && !density;  // J of D = L R^T from the half-transformed factors
+    RR_REQUIRE(!fused_j || (lfac && r > 0 && k_out), RR_ERR_ARG,
+               "rr_ri_jk_factor: J without a density needs the factor and the exchange build");
+    if (j_out && density) {
       const double* dD = rr::in_buf(c, rr::WS_P, density, n * n);
       jsplit = rr::coulomb_real_prec(c, t, precision, dD, &jpart);
     }
@@ -1727,7 +1992,19 @@ int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double
         hipLaunchKernelGGL((rr::k_pack<double>), dim3(rr::grid_for(r * n)), dim3(256), 0, c.stream, dR, r, 1, r, r, n, n,
                            1.0, lt + r * n);
       RR_HIP(hipGetLastError());
-      ex = rr::exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n);
+      double* dfac = nullptr;
+      double* gamma = nullptr;
+      if (fused_j) {
+        const size_t rp = rr::roundup(r, 8);
+        const int nblk = dR ? 2 : 1;
+        dfac = static_cast<double*>(rr::wsbuf(c, rr::WS_DENS2, n * nblk * rp * sizeof(double)));
+        hipLaunchKernelGGL(rr::k_build_dfac, dim3(rr::grid_for(n * nblk * rp)), dim3(256), 0, c.stream,
+                           dR ? lt + r * n : lt, (const double*)nullptr, r, n, n, rp, nblk, 1.0, 0.0, dfac);
+        RR_HIP(hipGetLastError());
+        gamma = static_cast<double*>(rr::wsbuf(c, rr::WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
+      }
+      ex = rr::exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma);
+      if (fused_j) jsplit = rr::coulomb_from_gamma(c, t, precision, gamma, &jpart);
     }
     rr::OutBuf oj = rr::out_buf(c, rr::WS_OUT_J, j_out, n * n), ok = rr::out_buf(c, rr::WS_OUT_K, k_out, n * n);
     if (k_out && r == 0) hipLaunchKernelGGL(rr::k_zero, dim3(rr::grid_for(n * n)), dim3(256), 0, c.stream, ok.dev, n * n);
@@ -1743,6 +2020,7 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
                       double* k_out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == nl, RR_ERR_DIM, "nonRelFockMatrix (RI): density_matrix dimension does not match nl");
     RR_REQUIRE(density && h_core && x_large && overlap, RR_ERR_ARG, "nonRelFockMatrix (RI): null input");
@@ -1776,6 +2054,7 @@ int rr_ri_jk_delta(rr_handle h, int slot, int precision, size_t n, size_t n_occ,
                    const double* c_old, double* dj_out, double* dk_out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == n, RR_ERR_DIM, "rr_ri_jk_delta: orbital dimension does not match the tensor");
     RR_REQUIRE(c_new && c_old, RR_ERR_ARG, "rr_ri_jk_delta: null orbitals");
@@ -1792,6 +2071,7 @@ int rr_x2c_fock_ri(rr_handle h, int slot, int precision, size_t nl, const double
                    size_t nx, const double* overlap, const double* density, double* fock) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == nl, RR_ERR_DIM, "x2cFockMatrix (RI): density_matrix dimension does not match 2*nLarge");
     RR_REQUIRE(h_x2c && x_large && overlap && density, RR_ERR_ARG, "x2cFockMatrix (RI): null input");
@@ -1829,6 +2109,7 @@ int rr_ri_fock_two_electron_direct(rr_handle h, int slot_ll, int slot_ss, int pr
                                    const double* occupations, double* g_out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& tl = rr::get_slot(c, slot_ll);
     rr::Tensor& ts = rr::get_slot(c, slot_ss);
     RR_REQUIRE(tl.naux == ts.naux, RR_ERR_DIM,
@@ -1859,6 +2140,7 @@ int rr_rkb_fock_ri(rr_handle h, int slot_ll, int slot_ss, int precision, size_t 
                    const double* density, double* fock) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& tl = rr::get_slot(c, slot_ll);
     rr::Tensor& ts = rr::get_slot(c, slot_ss);
     RR_REQUIRE(tl.naux == ts.naux, RR_ERR_DIM,
@@ -1907,6 +2189,7 @@ int rr_build_ri_mo_eri(rr_handle h, int slot_ll, int slot_ss, int precision, int
                        const double* c_ukb, size_t n_total, void* out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     RR_REQUIRE(out_precision == RR_FP64 || out_precision == RR_FP32, RR_ERR_ARG, "buildRiMoEri: output precision");
     RR_REQUIRE(!(precision == RR_FP64 && out_precision == RR_FP32), RR_ERR_ARG,
                "buildRiMoEri: FP64 arithmetic writes an FP64 tensor");
@@ -1939,6 +2222,7 @@ int rr_ri_mo_eri_rotated(rr_handle h, int precision, size_t n_total, size_t n_au
                          double* out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     RR_REQUIRE(b && u && out, RR_ERR_ARG, "RiMoEri::rotated: null input");
     const size_t cnt = 2 * n_aux * n_total * n_total;
     const double* db = rr::in_buf(c, rr::WS_P, b, cnt);
@@ -1969,6 +2253,7 @@ int rr_build_ri_nonrel_spin_mo_eri(rr_handle h, int slot, int precision, size_t 
                                    double* out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == n, RR_ERR_DIM, "buildRiNonRelSpinMoEri: eri3_L column count does not match n_spatial*n_spatial");
     RR_REQUIRE(c_spatial && out, RR_ERR_ARG, "buildRiNonRelSpinMoEri: null input");
@@ -1985,6 +2270,7 @@ int rr_ri_nonrel_spin_mo_eri_rotated(rr_handle h, int precision, size_t n_spatia
                                      const double* u, double* out) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     RR_REQUIRE(b && u && out, RR_ERR_ARG, "RiNonRelSpinMoEri::rotated: null input");
     const size_t n = n_spatial, n2 = 2 * n;
     std::vector<double> uh(n2 * n2);
@@ -2033,6 +2319,7 @@ int rr_nonrel_fock_cholesky(rr_handle h, int slot, int precision, size_t n, cons
                             double* fock) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == n, RR_ERR_DIM, "nonRelFockMatrix: AO Cholesky dimension does not match");
     RR_REQUIRE(h_core && p, RR_ERR_ARG, "nonRelFockMatrix: density dimensions inconsistent");
@@ -2057,6 +2344,7 @@ int rr_x2c_fock_cholesky(rr_handle h, int slot, int precision, size_t n, const d
                          double* fock) {
   return guarded(h, [&](Context& c) {
     rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
     rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == n, RR_ERR_DIM, "x2cFockMatrix: AO Cholesky dimension does not match");
     RR_REQUIRE(h_x2c && p, RR_ERR_ARG, "x2cFockMatrix: density dimensions inconsistent");

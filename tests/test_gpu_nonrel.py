@@ -125,13 +125,36 @@ def test_tile_configurations_agree_bitwise(ctx):
     ctx.upload(6, b, rr.COPY_FP64 | rr.COPY_FP32)
     d = syn.density(c)
     l = np.sqrt(2.0) * c
-    outs = []
-    for cfg in range(8):
-        ctx.set_tile_config(cfg)
-        outs.append(ctx.ri_jk_factor(6, d, l, precision=rr.FP64)[1])
-    ctx.set_tile_config(-1)
-    for o in outs[1:]:
-        assert np.array_equal(o, outs[0])
+    try:
+        for prec in (rr.FP64, rr.FP32):
+            outs = []
+            for cfg in list(range(8)) + list(range(100, 107)):
+                ctx.set_tile_config(cfg)
+                outs.append(ctx.ri_jk_factor(6, d, l, precision=prec)[1])
+            ctx.set_tile_config(-1)
+            outs.append(ctx.ri_jk_factor(6, d, l, precision=prec)[1])
+            for o in outs[1:]:
+                assert np.array_equal(o, outs[0])
+    finally:
+        ctx.set_tile_config(-1)
+
+
+@pytest.mark.parametrize("prec", [rr.FP64, rr.FP32, rr.BF16])
+def test_fused_coulomb_from_factors(ctx, oracle, prec):
+    # density=None: J of D = L R^T taken from the half-transformed factors inside the exchange GEMM
+    naux, n, r = 150, 70, 13
+    b = syn.ri_tensor(naux, n, 1.0, 10.0, 121)
+    b[:, 3, 5] += 0.2
+    g = syn.rng(122)
+    l, rf = g.standard_normal((n, r)), g.standard_normal((n, r))
+    ctx.upload(8, b, rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16)
+    j, k = ctx.ri_jk_factor(8, None, l, precision=prec)
+    jr, kr = oracle.ri_jk_factor(b.reshape(naux, -1), l @ l.T, l)
+    tol = {rr.FP64: 1e-12, rr.FP32: 2e-6, rr.BF16: 2e-2}[prec]
+    assert rel(j, jr) < tol and rel(k, kr) < tol
+    j2, k2 = ctx.ri_jk_factor(8, None, l, rf, precision=prec)
+    jr2, kr2 = oracle.ri_jk_factor(b.reshape(naux, -1), l @ rf.T, l, rf)
+    assert rel(j2, jr2) < tol and rel(k2, kr2) < tol
 
 
 def test_empty_occupied_space(ctx):
