@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--delta-precision", default="bf16", choices=["bf16", "f32", "f64"])
     p.add_argument("--cpu-sample", type=int, default=1500, help="aux rows timed on the CPU (0: skip)")
     p.add_argument("--no-fp64-leg", action="store_true")
+    p.add_argument("--copy-probe-gib", type=float, default=0.0,
+                   help="only run a device copy of this many GiB (calibrates HBM byte counters) and exit")
     return p.parse_args()
 
 
@@ -65,13 +67,41 @@ def orbital_sequence(n, nocc, nsteps, seed):
     return cs
 
 
+def copy_probe(torch, gib):
+    """A plain device-to-device copy of known size: under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it
+    gives the factor between the counters and the bytes actually moved."""
+    nbytes = int(gib * (1 << 30)) // 16 * 16
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
+    return 0
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (two rocprofv3 --pmc
+    runs, FETCH_SIZE and WRITE_SIZE, each scaled by the copy-probe calibration), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_large_real.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel + ":":
+            return float(parts[1])
+    return None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (the path would shard the "
-              "aux index over GPUs with one all-reduce of J and K per build; see DESIGN.md).", file=sys.stderr)
+              "aux index over GPUs with one all-reduce of J and K per build).", file=sys.stderr)
         return 2
     import torch
+    if a.copy_probe_gib > 0:
+        return copy_probe(torch, a.copy_probe_gib)
     import rerdmft_amd as rr
 
     dev = torch.device("cuda:0")
@@ -164,10 +194,19 @@ def main():
         """(work per launch, unit, bound) for a phase; full builds use rank n_occ, incremental 2 n_occ."""
         nb = prec_bytes[pname]
         if phase == "exchange_half":
-            per = 2.0 * naux * n * n * nocc * (2 if launches_delta else 1)
-            return per, "TFLOP/s", "mfma"
+            # flops 2 n_aux n^2 r; HBM bytes: B read once + W written once. The binding roofline is
+            # the one with the larger time at peak (BF16 sits just past the ridge: HBM-bound).
+            r = nocc * (2 if launches_delta else 1)
+            flops = 2.0 * naux * n * n * r
+            nbytes = (naux * n * n + naux * n * r) * nb
+            if nbytes / (PEAK["hbm"] * 1e9) > flops / (PEAK[pname] * 1e12):
+                return nbytes, "GB/s", "hbm"
+            return flops, "TFLOP/s", "mfma"
         if phase == "exchange_rank_k":
-            return 2.0 * naux * n * n * nocc, "TFLOP/s", "mfma"
+            # full build: symmetric K, n(n+1)/2 unique elements; incremental: G (non-symmetric) n^2
+            if launches_delta:
+                return 2.0 * naux * n * n * nocc, "TFLOP/s", "mfma"
+            return 1.0 * naux * n * (n + 1) * nocc, "TFLOP/s", "mfma"
         if phase == "coulomb":
             return 1.0 * naux * n * n * nb, "GB/s", "hbm"
         if phase == "gamma":
@@ -187,7 +226,7 @@ def main():
         achieved = work / (avg_ms * 1e-3) / 1e9
         peak = PEAK["hbm"]
     roof = {"kernel": f"{dom_phase}[{dom_phase_prec}]", "bound": bound, "achieved": achieved, "peak": peak,
-            "unit": unit, "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_ms,
+            "unit": unit, "frac": achieved / peak, "traffic": read_traffic(f"{dom_phase}[{dom_phase_prec}]"), "avg_launch_ms": avg_ms,
             "algorithmic_per_launch": work, "launches": dom_cnt}
 
     # per-kernel rooflines of the plain FP64 build (for the record)
