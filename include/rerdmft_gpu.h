@@ -140,6 +140,75 @@ int rr_cholesky_to_mo_spatial(rr_handle h, int slot, int precision, size_t n, co
 int rr_cholesky_to_mo_spinor(rr_handle h, int slot, int precision, size_t n, const double* c, size_t n_mo,
                              double* out);
 
+/* ---- Functional stage: RI generalized Fock and sigma-vector engine on MO RI tensors ---------
+ * b is an MO RI tensor (n_aux, n*n), B(P,p,q) at [P*n*n + p*n + q], real or complex interleaved
+ * (is_complex), with eri(a,b,c,d) = sum_P B(P,a,c) B(P,b,d). Couplings two_rdm_* are real n x n;
+ * pair_of (optional, length n) is the pair-partner involution. Host or device pointers. */
+
+/* riHxFockGemm: src/Hessian_opt/RiGemmFock.cpp:53-115, behind hartreeExchangeFockMatrix /
+ * jkOnlyFockMatrix for RiMoEri and RiNonRelSpinMoEri (:119-163; pass embeddedB for the latter).
+ * F(p,q) = occ_q h(q,p) + sum_s [h2(s,q) eri(q,s,p,s) - x2(q,s) eri(q,s,s,p)]
+ *          + [pair_of] 2 sum_s eri(s,sbar,qbar,p) (l1(s,qbar) + l2(s,q)). */
+int rr_ri_hx_fock_gemm(rr_handle h, int is_complex, size_t n, size_t n_aux, const double* b, const double* h_mo,
+                       const double* occupations, const double* two_rdm_h, const double* two_rdm_x,
+                       const size_t* pair_of, const double* two_rdm_l1, const double* two_rdm_l2, double* fock);
+
+/* buildRiSigmaHessianToolkit: src/Hessian_opt/RiSigmaHessianToolkit.cpp:25-56 (struct
+ * RiSigmaHessianToolkit.h:44-61). kappa real n x n. Outputs (any may be NULL): kb1, kb2 (n_aux, n*n);
+ * bdiag, kb1diag, kb2diag (n_aux, n). */
+int rr_ri_sigma_toolkit(rr_handle h, int is_complex, size_t n, size_t n_aux, const double* b, const double* kappa,
+                        double* kb1, double* kb2, double* bdiag, double* kb1diag, double* kb2diag);
+
+/* Operand of a pair contraction (risigma::PairOp, src/Hessian_opt/RiSigmaGemm.h:23-45):
+ *   value(P,t,x) = [s ? s(P, idx[t*n+x]) : 1] * [w ? w[t*n+x] : 1] * [row ? row(P,t) : 1],
+ * right-multiplied by post (real n x n) when given. s (n_aux, n*n) and row (n_aux, n) have the
+ * element type of the call; idx (uint32 n*n) is required with s. */
+typedef struct rr_pair_op {
+  const double* s;
+  const uint32_t* idx;
+  const double* w;
+  const double* row;
+  const double* post;
+} rr_pair_op;
+
+/* pairContract: src/Hessian_opt/RiSigmaGemm.cpp:76-101.
+ * target(x,y) += sign * sum_{P,t} F(P,t,x) G(P,t,y); target n x n (in/out). */
+int rr_ri_pair_contract(rr_handle h, int is_complex, size_t n, size_t n_aux, double sign, const rr_pair_op* f,
+                        const rr_pair_op* g, double* target);
+/* diagContract: src/Hessian_opt/RiSigmaGemm.cpp:103-121.
+ * target(a,b) += sum_P S(P, b*n+a) (Ua(P,a) + Ub(P,b)); ua/ub (n_aux, n) may be NULL. */
+int rr_ri_diag_contract(rr_handle h, int is_complex, size_t n, size_t n_aux, const double* s, const double* ua,
+                        const double* ub, double* target);
+/* accumU: src/Hessian_opt/RiSigmaGemm.cpp:123-132. U(P,x) += alpha sum_t D[P*n+t] Wm(x,t);
+ * d, u (n_aux, n) of the call's element type, wm real n x n. */
+int rr_ri_accum_u(rr_handle h, int is_complex, size_t n, size_t n_aux, double alpha, const double* d, const double* wm,
+                  double* u);
+
+/* Sigma-vector orbital-Hessian products w = H v. pairs: n_pairs (p,q) index pairs, flattened.
+ * jkOnlySigmaHessianVectorRi: src/Hessian_opt/JkOnlySigmaHessianVectorRi.cpp:504-523 (real). */
+int rr_jk_only_sigma_hessian_vector_ri(rr_handle h, size_t n, const double* h_mo, const double* b, size_t n_aux,
+                                       const double* occupations, const double* two_rdm_h, const double* two_rdm_x,
+                                       size_t n_pairs, const size_t* pairs, const double* v, double* w);
+/* jkOnlySigmaJointHessianVectorRi: JkOnlySigmaHessianVectorRi.cpp:525-573 (complex h, b;
+ * v and w have 2*n_pairs entries [t; y]). */
+int rr_jk_only_sigma_joint_hessian_vector_ri(rr_handle h, size_t n, const double* h_mo, const double* b, size_t n_aux,
+                                             const double* occupations, const double* two_rdm_h,
+                                             const double* two_rdm_x, size_t n_pairs, const size_t* pairs,
+                                             const double* v, double* w);
+/* hartreeExchangeSigmaHessianVectorRi: src/Hessian_opt/HartreeExchangeSigmaHessianVectorRi.cpp:442-484. */
+int rr_hartree_exchange_sigma_hessian_vector_ri(rr_handle h, size_t n, const double* h_mo, const double* b,
+                                                size_t n_aux, const double* occupations, const double* two_rdm_h,
+                                                const double* two_rdm_x, const double* fock, const size_t* pair_of,
+                                                const double* two_rdm_l1, const double* two_rdm_l2, size_t n_pairs,
+                                                const size_t* pairs, const double* v, double* w);
+/* hartreeExchangeSigmaJointHessianVectorRi: HartreeExchangeSigmaHessianVectorRi.cpp:486-539. */
+int rr_hartree_exchange_sigma_joint_hessian_vector_ri(rr_handle h, size_t n, const double* h_mo, const double* b,
+                                                      size_t n_aux, const double* occupations, const double* two_rdm_h,
+                                                      const double* two_rdm_x, const double* fock,
+                                                      const size_t* pair_of, const double* two_rdm_l1,
+                                                      const double* two_rdm_l2, size_t n_pairs, const size_t* pairs,
+                                                      const double* v, double* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -202,3 +202,109 @@ def eig_hermitian(a):
     v = np.empty((n, n), dtype=np.complex128)
     _check(lib().oracle_eig_hermitian(_sz(n), _p(a), _p(w), _p(v)))
     return w, v
+
+
+# ---- functional stage (rr_oracle_sigma.cpp) --------------------------------------------------------
+def _cx(a, cplx):
+    return _c128(a) if cplx else _f64(a)
+
+
+def _check_sigma(rc):
+    if rc != 0:
+        raise RuntimeError(lib().oracle_sigma_last_error().decode())
+
+
+def ri_hx_fock_gemm(b, h, occ, h2, x2, pair_of=None, l1=None, l2=None):
+    """riHxFockGemm (src/Hessian_opt/RiGemmFock.cpp:53-115)."""
+    cplx = np.iscomplexobj(b) or np.iscomplexobj(h)
+    n, naux = h.shape[0], b.shape[0]
+    bb, hh = _cx(b, cplx), _cx(h, cplx)
+    oc, a2, b2 = _f64(occ), _f64(h2), _f64(x2)
+    pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.int64))
+    l1a, l2a = (None, None) if pair_of is None else (_f64(l1), _f64(l2))
+    f = np.zeros((n, n), dtype=np.complex128 if cplx else np.float64)
+    lib().oracle_sigma_last_error.restype = ctypes.c_char_p
+    _check_sigma(lib().oracle_ri_hx_fock_gemm(ctypes.c_int(int(cplx)), _sz(n), _sz(naux), _p(bb), _p(hh), _p(oc),
+                                              _p(a2), _p(b2), _p(pa), _p(l1a), _p(l2a), _p(f)))
+    return f
+
+
+def ri_sigma_toolkit(b, kappa):
+    """buildRiSigmaHessianToolkit (src/Hessian_opt/RiSigmaHessianToolkit.cpp:25-56)."""
+    cplx = np.iscomplexobj(b)
+    naux, n = b.shape[0], kappa.shape[0]
+    dt = np.complex128 if cplx else np.float64
+    bb, kk = _cx(b, cplx), _f64(kappa)
+    kb1, kb2 = np.zeros((naux, n, n), dt), np.zeros((naux, n, n), dt)
+    bd, k1, k2 = np.zeros((naux, n), dt), np.zeros((naux, n), dt), np.zeros((naux, n), dt)
+    lib().oracle_ri_sigma_toolkit(ctypes.c_int(int(cplx)), _sz(n), _sz(naux), _p(bb), _p(kk), _p(kb1), _p(kb2),
+                                  _p(bd), _p(k1), _p(k2))
+    return kb1, kb2, bd, k1, k2
+
+
+def ri_pair_contract(target, sign, n, naux, f, g):
+    """pairContract (src/Hessian_opt/RiSigmaGemm.cpp:76-101); f, g dicts (s, idx, w, row, post)."""
+    cplx = np.iscomplexobj(target)
+    keep = []
+
+    def fields(d):
+        out = []
+        for key in ("s", "idx", "w", "row", "post"):
+            v = d.get(key)
+            if v is None:
+                out.append(ctypes.c_void_p(0))
+                continue
+            dt = np.uint32 if key == "idx" else ((np.complex128 if cplx else np.float64) if key in ("s", "row")
+                                                 else np.float64)
+            a = np.ascontiguousarray(v, dtype=dt)
+            keep.append(a)
+            out.append(_p(a))
+        return out
+
+    t = _cx(target, cplx)
+    lib().oracle_ri_pair_contract(ctypes.c_int(int(cplx)), _sz(n), _sz(naux), ctypes.c_double(sign), *fields(f),
+                                  *fields(g), _p(t))
+    return t
+
+
+def ri_diag_contract(target, s, ua=None, ub=None):
+    """diagContract (src/Hessian_opt/RiSigmaGemm.cpp:103-121)."""
+    cplx = np.iscomplexobj(target)
+    n, naux = target.shape[0], s.shape[0]
+    t = _cx(target, cplx)
+    ss = _cx(s, cplx)
+    a = None if ua is None else _cx(ua, cplx)
+    b_ = None if ub is None else _cx(ub, cplx)
+    lib().oracle_ri_diag_contract(ctypes.c_int(int(cplx)), _sz(n), _sz(naux), _p(ss), _p(a), _p(b_), _p(t))
+    return t
+
+
+def ri_accum_u(u, alpha, d, wm):
+    """accumU (src/Hessian_opt/RiSigmaGemm.cpp:123-132)."""
+    cplx = np.iscomplexobj(u)
+    naux, n = u.shape
+    uu, dd, ww = _cx(u, cplx), _cx(d, cplx), _f64(wm)
+    lib().oracle_ri_accum_u(ctypes.c_int(int(cplx)), _sz(n), _sz(naux), ctypes.c_double(alpha), _p(dd), _p(ww), _p(uu))
+    return uu
+
+
+def sigma_hessian_vector_ri(h, b, occ, hc, xc, pairs, v, fock=None, pair_of=None, l1=None, l2=None, hx=False):
+    """jkOnlySigma(Joint)HessianVectorRi (JkOnlySigmaHessianVectorRi.cpp:504-573) and
+    hartreeExchangeSigma(Joint)HessianVectorRi (HartreeExchangeSigmaHessianVectorRi.cpp:442-539).
+    Complex h/b select the joint [t; y] form."""
+    cplx = np.iscomplexobj(b) or np.iscomplexobj(h)
+    n, naux = h.shape[0], b.shape[0]
+    pp = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint64).reshape(-1, 2))
+    npairs = pp.shape[0]
+    hh, bb = _cx(h, cplx), _cx(b, cplx)
+    oc, a, x = _f64(occ), _f64(hc), _f64(xc)
+    ff = None if fock is None else _cx(fock, cplx)
+    pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.int64))
+    l1a, l2a = (None, None) if pair_of is None else (_f64(l1), _f64(l2))
+    vv = _f64(v)
+    w = np.zeros(npairs * (2 if cplx else 1))
+    lib().oracle_sigma_last_error.restype = ctypes.c_char_p
+    _check_sigma(lib().oracle_sigma_hessian_vector_ri(ctypes.c_int(int(cplx)), ctypes.c_int(int(hx)), _sz(n), _p(hh),
+                                                      _p(bb), _sz(naux), _p(oc), _p(a), _p(x), _p(ff), _p(pa),
+                                                      _p(l1a), _p(l2a), _sz(npairs), _p(pp), _p(vv), _p(w)))
+    return w

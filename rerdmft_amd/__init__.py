@@ -30,7 +30,16 @@ EXPORTED = [
     "rr_ri_mo_eri_positive_block", "rr_build_ri_nonrel_spin_mo_eri", "rr_ri_nonrel_spin_mo_eri_rotated",
     "rr_ri_nonrel_spin_embedded_b", "rr_nonrel_fock_cholesky", "rr_x2c_fock_cholesky",
     "rr_cholesky_to_mo_spatial", "rr_cholesky_to_mo_spinor",
+    "rr_ri_hx_fock_gemm", "rr_ri_sigma_toolkit", "rr_ri_pair_contract", "rr_ri_diag_contract", "rr_ri_accum_u",
+    "rr_jk_only_sigma_hessian_vector_ri", "rr_jk_only_sigma_joint_hessian_vector_ri",
+    "rr_hartree_exchange_sigma_hessian_vector_ri", "rr_hartree_exchange_sigma_joint_hessian_vector_ri",
 ]
+
+
+class PairOp(ctypes.Structure):
+    """rr_pair_op (risigma::PairOp): value(P,t,x) = s(P, idx[t*n+x]) * w[t*n+x] * row(P,t), then @ post."""
+    _fields_ = [("s", ctypes.c_void_p), ("idx", ctypes.c_void_p), ("w", ctypes.c_void_p), ("row", ctypes.c_void_p),
+                ("post", ctypes.c_void_p)]
 
 _lib = None
 
@@ -336,3 +345,129 @@ class Context:
         self._call("rr_cholesky_to_mo_spinor", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), cc.ptr, _sz(nmo),
                    op)
         return o
+
+    # ---- functional stage: MO RI tensors (RiGemmFock / RiSigma*) ---------------------------------
+    @staticmethod
+    def _elem(b):
+        return np.complex128 if np.iscomplexobj(b) else np.float64
+
+    def ri_hx_fock_gemm(self, b, h_mo, occupations, two_rdm_h, two_rdm_x, pair_of=None, two_rdm_l1=None,
+                        two_rdm_l2=None, out=None):
+        """riHxFockGemm -- src/Hessian_opt/RiGemmFock.cpp:53-115 (hartreeExchangeFockMatrix /
+        jkOnlyFockMatrix on an RI MO tensor b of shape (n_aux, n, n) or (n_aux, n*n))."""
+        cplx = np.iscomplexobj(b) or np.iscomplexobj(h_mo)
+        dt = np.complex128 if cplx else np.float64
+        n = h_mo.shape[0]
+        naux = b.shape[0]
+        bb, hh = _Arr(b, dt), _Arr(h_mo, dt)
+        oc, h2, x2 = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
+        l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+        o, op = _out(out, (n, n), dt)
+        self._call("rr_ri_hx_fock_gemm", ctypes.c_int(int(cplx)), _sz(n), _sz(naux), bb.ptr, hh.ptr, oc.ptr, h2.ptr,
+                   x2.ptr, ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr, op)
+        return o
+
+    def ri_sigma_toolkit(self, b, kappa):
+        """buildRiSigmaHessianToolkit -- src/Hessian_opt/RiSigmaHessianToolkit.cpp:25-56.
+        Returns (kb1, kb2, bdiag, kb1diag, kb2diag) as numpy arrays."""
+        dt = self._elem(b)
+        naux, n = b.shape[0], kappa.shape[0]
+        bb, kk = _Arr(b, dt), _f64(kappa)
+        kb1, p1 = _out(None, (naux, n, n), dt)
+        kb2, p2 = _out(None, (naux, n, n), dt)
+        bd, p3 = _out(None, (naux, n), dt)
+        k1d, p4 = _out(None, (naux, n), dt)
+        k2d, p5 = _out(None, (naux, n), dt)
+        self._call("rr_ri_sigma_toolkit", ctypes.c_int(int(dt == np.complex128)), _sz(n), _sz(naux), bb.ptr, kk.ptr,
+                   p1, p2, p3, p4, p5)
+        return kb1, kb2, bd, k1d, k2d
+
+    def ri_pair_contract(self, target, sign, n, naux, f, g):
+        """pairContract -- src/Hessian_opt/RiSigmaGemm.cpp:76-101. f, g: dicts with optional keys
+        s (n_aux, n, n), idx (n, n) uint32, w (n, n), row (n_aux, n), post (n, n). target updated in place."""
+        dt = np.complex128 if np.iscomplexobj(target) else np.float64
+        keep = []
+
+        def mk(d):
+            op = PairOp()
+            for key in ("s", "idx", "w", "row", "post"):
+                v = d.get(key)
+                if v is None:
+                    continue
+                kdt = np.uint32 if key == "idx" else (dt if key in ("s", "row") else np.float64)
+                a = np.ascontiguousarray(v, dtype=kdt)
+                keep.append(a)
+                setattr(op, key, a.ctypes.data)
+            return op
+
+        fo, go = mk(f), mk(g)
+        t = np.ascontiguousarray(target, dtype=dt)
+        self._call("rr_ri_pair_contract", ctypes.c_int(int(dt == np.complex128)), _sz(n), _sz(naux),
+                   ctypes.c_double(sign), ctypes.byref(fo), ctypes.byref(go), ctypes.c_void_p(t.ctypes.data))
+        target[...] = t
+        return target
+
+    def ri_diag_contract(self, target, s, ua=None, ub=None):
+        """diagContract -- src/Hessian_opt/RiSigmaGemm.cpp:103-121 (target updated in place)."""
+        dt = np.complex128 if np.iscomplexobj(target) else np.float64
+        n, naux = target.shape[0], s.shape[0]
+        ss, a, b_ = _Arr(s, dt), _Arr(ua, dt), _Arr(ub, dt)
+        t = np.ascontiguousarray(target, dtype=dt)
+        self._call("rr_ri_diag_contract", ctypes.c_int(int(dt == np.complex128)), _sz(n), _sz(naux), ss.ptr, a.ptr,
+                   b_.ptr, ctypes.c_void_p(t.ctypes.data))
+        target[...] = t
+        return target
+
+    def ri_accum_u(self, u, alpha, d, wm):
+        """accumU -- src/Hessian_opt/RiSigmaGemm.cpp:123-132 (u updated in place)."""
+        dt = np.complex128 if np.iscomplexobj(u) else np.float64
+        naux, n = u.shape
+        dd, ww = _Arr(d, dt), _f64(wm)
+        t = np.ascontiguousarray(u, dtype=dt)
+        self._call("rr_ri_accum_u", ctypes.c_int(int(dt == np.complex128)), _sz(n), _sz(naux), ctypes.c_double(alpha),
+                   dd.ptr, ww.ptr, ctypes.c_void_p(t.ctypes.data))
+        u[...] = t
+        return u
+
+    def _sigma(self, name, cplx, h_mo, b, occupations, two_rdm_h, two_rdm_x, pairs, v, fock=None, pair_of=None,
+               two_rdm_l1=None, two_rdm_l2=None, hx=False):
+        dt = np.complex128 if cplx else np.float64
+        n, naux = h_mo.shape[0], b.shape[0]
+        pp = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint64).reshape(-1, 2))
+        npairs = pp.shape[0]
+        vv = _f64(v)
+        w = np.zeros(npairs * (2 if cplx else 1))
+        hh, bb = _Arr(h_mo, dt), _Arr(b, dt)
+        oc, hc, xc = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        args = [_sz(n), hh.ptr, bb.ptr, _sz(naux), oc.ptr, hc.ptr, xc.ptr]
+        if hx:
+            ff = _Arr(fock, dt)
+            pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
+            l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+            args += [ff.ptr, ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr]
+        args += [_sz(npairs), ctypes.c_void_p(pp.ctypes.data), vv.ptr, ctypes.c_void_p(w.ctypes.data)]
+        self._call(name, *args)
+        return w
+
+    def jk_only_sigma_hessian_vector_ri(self, h_mo, b, occupations, two_rdm_h, two_rdm_x, pairs, v):
+        """jkOnlySigmaHessianVectorRi -- src/Hessian_opt/JkOnlySigmaHessianVectorRi.cpp:504-523."""
+        return self._sigma("rr_jk_only_sigma_hessian_vector_ri", False, h_mo, b, occupations, two_rdm_h, two_rdm_x,
+                           pairs, v)
+
+    def jk_only_sigma_joint_hessian_vector_ri(self, h_mo, b, occupations, two_rdm_h, two_rdm_x, pairs, v):
+        """jkOnlySigmaJointHessianVectorRi -- JkOnlySigmaHessianVectorRi.cpp:525-573."""
+        return self._sigma("rr_jk_only_sigma_joint_hessian_vector_ri", True, h_mo, b, occupations, two_rdm_h, two_rdm_x,
+                           pairs, v)
+
+    def hartree_exchange_sigma_hessian_vector_ri(self, h_mo, b, occupations, two_rdm_h, two_rdm_x, fock, pairs, v,
+                                                 pair_of=None, two_rdm_l1=None, two_rdm_l2=None):
+        """hartreeExchangeSigmaHessianVectorRi -- HartreeExchangeSigmaHessianVectorRi.cpp:442-484."""
+        return self._sigma("rr_hartree_exchange_sigma_hessian_vector_ri", False, h_mo, b, occupations, two_rdm_h,
+                           two_rdm_x, pairs, v, fock, pair_of, two_rdm_l1, two_rdm_l2, hx=True)
+
+    def hartree_exchange_sigma_joint_hessian_vector_ri(self, h_mo, b, occupations, two_rdm_h, two_rdm_x, fock, pairs,
+                                                       v, pair_of=None, two_rdm_l1=None, two_rdm_l2=None):
+        """hartreeExchangeSigmaJointHessianVectorRi -- HartreeExchangeSigmaHessianVectorRi.cpp:486-539."""
+        return self._sigma("rr_hartree_exchange_sigma_joint_hessian_vector_ri", True, h_mo, b, occupations, two_rdm_h,
+                           two_rdm_x, pairs, v, fock, pair_of, two_rdm_l1, two_rdm_l2, hx=True)

@@ -9,8 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librerdmft_gpu.so")
-SOURCES = [os.path.join(HERE, "csrc", "rr_lib.hip")]
-HEADERS = [os.path.join(HERE, "csrc", f) for f in ("rr_device.hpp", "rr_gemm.hpp")] + [
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rr_lib.hip", "rr_sigma.hip")]
+HEADERS = [os.path.join(HERE, "csrc", f) for f in ("rr_device.hpp", "rr_gemm.hpp", "rr_core.hpp")] + [
     os.path.join(ROOT, "include", "rerdmft_gpu.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -29,7 +29,22 @@ LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib
 
 
 def build_library(verbose=False):
-    cmd = [HIPCC] + FLAGS + SOURCES + ["-o", LIB + ".tmp"] + LIBS
+    """Compiles each translation unit in parallel, then links the shared library."""
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(os.path.basename(src))[0] + ".o")
+        cmd = [HIPCC] + compile_flags + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), cmd))
+        objs.append(obj)
+    failed = [cmd for p, cmd in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-o", LIB + ".tmp"] + LIBS
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

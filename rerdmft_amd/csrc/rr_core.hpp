@@ -1,0 +1,170 @@
+// Internal interface shared by the translation units of librerdmft_gpu (not installed; the public
+// interface is include/rerdmft_gpu.h). Each translation unit has its own kernels; what is shared
+// here are the context, error handling, workspace management and the host-side GEMM / small-matrix
+// drivers.
+#pragma once
+
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "rr_gemm.hpp"
+#include "../../include/rerdmft_gpu.h"
+
+namespace rr {
+
+
+// ================================================================================================
+// Context, errors, workspace
+// ================================================================================================
+struct Tensor {
+  size_t naux = 0, n = 0, ld = 0;
+  double* f64 = nullptr;
+  float* f32 = nullptr;
+  bf16_t* b16 = nullptr;
+  bool symmetric = true;
+};
+
+constexpr int kSlots = 16;
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::string err;
+  Tensor slot[kSlots];
+  std::map<int, std::pair<void*, size_t>> ws;
+  int tile_override = -1;
+  int lds_override = -1;  // -1 auto, -2 generic kernel only, >= 0 forced LDS variant
+  bool autotune = true;
+  int last_ntiles = 0;  // tiles per batch of the most recent GEMM launch
+  std::map<std::vector<long long>, std::pair<int, int>> tuned;  // shape key -> (generic cfg, LDS variant)
+  float last_ms[8] = {0};
+};
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define RR_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw Error{RR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)};             \
+  } while (0)
+
+#define RR_REQUIRE(cond, code, msg) \
+  do {                              \
+    if (!(cond)) throw Error{code, msg}; \
+  } while (0)
+
+inline size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+
+// Named grow-only device workspace buffers.
+void* wsbuf(Context& c, int id, size_t bytes);
+
+enum WsId {
+  WS_STAGE = 1, WS_DT, WS_GPART, WS_GAMMA, WS_JPART, WS_LT, WS_W, WS_KPART, WS_OUT_F, WS_OUT_J, WS_OUT_K,
+  WS_H, WS_S, WS_X, WS_P, WS_T1, WS_T2, WS_T3, WS_T4, WS_EIGW, WS_EIGE, WS_INFO, WS_COCC, WS_W2, WS_CA, WS_CB,
+  WS_MO_A, WS_MO_B, WS_MO_S1, WS_MO_OUT, WS_DENS2, WS_TMP8, WS_TMP9
+};
+// Translation units other than rr_lib.hip number their workspace buffers from here.
+constexpr int WS_SIGMA_BASE = 100000;
+
+// True when p is device (or managed) memory visible to the current device.
+bool is_device_ptr(const void* p);
+
+// Copy host-or-device array into a device workspace buffer (returns device pointer).
+template <typename T>
+T* to_device(Context& c, int id, const T* src, size_t count) {
+  T* d = static_cast<T*>(wsbuf(c, id, count * sizeof(T)));
+  if (count) RR_HIP(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyDefault, c.stream));
+  return d;
+}
+template <typename T>
+void from_device(Context& c, T* dst, const T* src, size_t count) {
+  if (count) RR_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDefault, c.stream));
+}
+
+inline int grid_for(size_t total, int block = 256, int maxg = 65536) {
+  size_t g = (total + block - 1) / block;
+  if (g < 1) g = 1;
+  return (int)std::min<size_t>(g, maxg);
+}
+
+enum Prec { P64 = 0, P32 = 1, P16 = 2 };
+
+struct GemmSpec {
+  int prec = P64;      // input type
+  int out = P64;       // output type
+  bool cplx = false;
+  int cfg = 1;
+  int lds = -1;        // >= 0: LDS-staged kernel variant (real only), see kLds
+};
+
+// Tiled MFMA GEMM C = alpha A B^T + beta C on the context stream (rr_gemm.hpp semantics).
+void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g);
+// As run_gemm, with the per-shape tile autotuner for large products.
+void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g);
+size_t prec_size(int p);
+struct Tensor;
+const void* tensor_data(const Tensor& t, int prec);
+// Small FP64 (or complex FP64, interleaved) row-major product C = alpha op(A) op(B) + beta C,
+// op in {'N', 'T', 'C'}; pointers are device memory.
+void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, size_t k, const double* A, size_t lda,
+                 const double* B, size_t ldb, double* C, size_t ldc, double alpha = 1.0, double beta = 0.0);
+// Hermitian eigensolve in place (row j of a = eigenvector j), ascending eigenvalues in w.
+void eig_device(Context& c, bool cplx, size_t n, double* a, double* w);
+Tensor& get_slot(Context& c, int slot);
+void check_prec(int prec);
+
+// Output helper: device pointer if the caller's buffer is device memory, else a workspace buffer
+// that is copied back by finish_out().
+struct OutBuf {
+  double* dev = nullptr;
+  double* user = nullptr;
+  size_t count = 0;
+  bool copy = false;
+};
+OutBuf out_buf(Context& c, int id, double* user, size_t count);
+void finish_out(Context& c, const OutBuf& o);
+// Input helper: the caller's pointer if it is device memory, else a device copy in workspace `id`.
+const double* in_buf(Context& c, int id, const double* user, size_t count);
+
+}  // namespace rr
+
+struct rr_context {
+  rr::Context ctx;
+};
+
+namespace rr {
+// Runs f(ctx) with the device current; maps exceptions to status codes and keeps the message.
+template <typename F>
+int guarded(rr_handle h, F&& f) {
+  if (!h) return RR_ERR_ARG;
+  try {
+    RR_HIP(hipSetDevice(h->ctx.device));
+    f(h->ctx);
+    RR_HIP(hipStreamSynchronize(h->ctx.stream));
+    h->ctx.err.clear();
+    return RR_OK;
+  } catch (const Error& e) {
+    h->ctx.err = e.msg;
+    (void)hipStreamSynchronize(h->ctx.stream);
+    return e.code;
+  } catch (const std::exception& e) {
+    h->ctx.err = e.what();
+    return RR_ERR_INTERNAL;
+  }
+}
+}  // namespace rr

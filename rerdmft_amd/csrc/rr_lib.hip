@@ -8,7 +8,6 @@
 //   exchange half-transform  W[nu][(P,i)] = sum_mu Bt_P[nu][mu] Lt[i][mu]      (A = Bt_P, B = Lt)
 //   exchange build           K[mu][nu]    = sum_(P,i) W[mu][(P,i)] W[nu][(P,i)] (Hermitian rank-k)
 //   Coulomb                  gamma_P = <Bt_P, P^T>,  Jt = sum_P gamma_P Bt_P      (HBM streams)
-#include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -21,58 +20,9 @@
 #include <type_traits>
 #include <vector>
 
-#include "rr_gemm.hpp"
-#include "../../include/rerdmft_gpu.h"
+#include "rr_core.hpp"
 
 namespace rr {
-
-// ================================================================================================
-// Context, errors, workspace
-// ================================================================================================
-struct Tensor {
-  size_t naux = 0, n = 0, ld = 0;
-  double* f64 = nullptr;
-  float* f32 = nullptr;
-  bf16_t* b16 = nullptr;
-  bool symmetric = true;
-};
-
-constexpr int kSlots = 16;
-
-struct Context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  rocblas_handle blas = nullptr;
-  std::string err;
-  Tensor slot[kSlots];
-  std::map<int, std::pair<void*, size_t>> ws;
-  int tile_override = -1;
-  int lds_override = -1;  // -1 auto, -2 generic kernel only, >= 0 forced LDS variant
-  bool autotune = true;
-  int last_ntiles = 0;  // tiles per batch of the most recent GEMM launch
-  std::map<std::vector<long long>, std::pair<int, int>> tuned;  // shape key -> (generic cfg, LDS variant)
-  float last_ms[8] = {0};
-};
-
-struct Error {
-  int code;
-  std::string msg;
-};
-
-#define RR_HIP(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess)                                                                     \
-      throw Error{RR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)};             \
-  } while (0)
-
-#define RR_REQUIRE(cond, code, msg) \
-  do {                              \
-    if (!(cond)) throw Error{code, msg}; \
-  } while (0)
-
-inline size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
-
 // Named grow-only device workspace buffers.
 void* wsbuf(Context& c, int id, size_t bytes) {
   auto it = c.ws.find(id);
@@ -88,11 +38,6 @@ void* wsbuf(Context& c, int id, size_t bytes) {
   return p;
 }
 
-enum WsId {
-  WS_STAGE = 1, WS_DT, WS_GPART, WS_GAMMA, WS_JPART, WS_LT, WS_W, WS_KPART, WS_OUT_F, WS_OUT_J, WS_OUT_K,
-  WS_H, WS_S, WS_X, WS_P, WS_T1, WS_T2, WS_T3, WS_T4, WS_EIGW, WS_EIGE, WS_INFO, WS_COCC, WS_W2, WS_CA, WS_CB,
-  WS_MO_A, WS_MO_B, WS_MO_S1, WS_MO_OUT, WS_DENS2, WS_TMP8, WS_TMP9
-};
 
 // True when p is device (or managed) memory visible to the current device.
 bool is_device_ptr(const void* p) {
@@ -106,17 +51,6 @@ bool is_device_ptr(const void* p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-// Copy host-or-device array into a device workspace buffer (returns device pointer).
-template <typename T>
-T* to_device(Context& c, int id, const T* src, size_t count) {
-  T* d = static_cast<T*>(wsbuf(c, id, count * sizeof(T)));
-  if (count) RR_HIP(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyDefault, c.stream));
-  return d;
-}
-template <typename T>
-void from_device(Context& c, T* dst, const T* src, size_t count) {
-  if (count) RR_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDefault, c.stream));
-}
 
 // ================================================================================================
 // Elementwise / layout kernels
@@ -207,11 +141,6 @@ __global__ void k_zero(double* p, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
-inline int grid_for(size_t total, int block = 256, int maxg = 65536) {
-  size_t g = (total + block - 1) / block;
-  if (g < 1) g = 1;
-  return (int)std::min<size_t>(g, maxg);
-}
 
 // ================================================================================================
 // Coulomb streams
@@ -452,7 +381,6 @@ __global__ void k_reduce_parts(const double* part, int nsplit, size_t stride, si
 // ================================================================================================
 // GEMM dispatch (tile configurations instantiated for gfx950; the autotuner picks among them)
 // ================================================================================================
-enum Prec { P64 = 0, P32 = 1, P16 = 2 };
 
 struct TileCfg {
   int wm, wn;
@@ -481,13 +409,6 @@ void launch_gemm_cfg(int cfg, const GemmArgs& g, int grid, hipStream_t s) {
   }
 }
 
-struct GemmSpec {
-  int prec = P64;      // input type
-  int out = P64;       // output type
-  bool cplx = false;
-  int cfg = 1;
-  int lds = -1;        // >= 0: LDS-staged kernel variant (real only), see kLds
-};
 
 // LDS-staged kernel variants: (BM, BN, waves along M)
 struct LdsCfg {
@@ -681,7 +602,7 @@ __global__ void k_pack_op(const double* in, size_t ld_in, char op, int cplx, siz
 // C (m x n, row-major, ldc) = op(A) (m x k) * op(B) (k x n); A/B device pointers with their own
 // row-major leading dimensions. cplx: interleaved complex.
 void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, size_t k, const double* A, size_t lda,
-                 const double* B, size_t ldb, double* C, size_t ldc, double alpha = 1.0, double beta = 0.0) {
+                 const double* B, size_t ldb, double* C, size_t ldc, double alpha, double beta) {
   if (m == 0 || n == 0) return;
   const int cw = cplx ? 2 : 1;
   const size_t kp = roundup(std::max<size_t>(k, 1), 8);
@@ -1085,12 +1006,6 @@ void check_prec(int prec) { RR_REQUIRE(prec == P64 || prec == P32 || prec == P16
 
 // Output helper: device pointer if the caller's buffer is device memory, else a workspace buffer
 // that is copied back by finish_out().
-struct OutBuf {
-  double* dev = nullptr;
-  double* user = nullptr;
-  size_t count = 0;
-  bool copy = false;
-};
 OutBuf out_buf(Context& c, int id, double* user, size_t count) {
   OutBuf o;
   o.user = user, o.count = count;
@@ -1785,31 +1700,8 @@ __global__ void k_split_block(const double* p, size_t n, size_t s, size_t u, dou
 // ================================================================================================
 using rr::Context;
 using rr::Error;
+using rr::guarded;
 
-struct rr_context {
-  Context ctx;
-};
-
-namespace {
-template <typename F>
-int guarded(rr_handle h, F&& f) {
-  if (!h) return RR_ERR_ARG;
-  try {
-    RR_HIP(hipSetDevice(h->ctx.device));
-    f(h->ctx);
-    RR_HIP(hipStreamSynchronize(h->ctx.stream));
-    h->ctx.err.clear();
-    return RR_OK;
-  } catch (const Error& e) {
-    h->ctx.err = e.msg;
-    (void)hipStreamSynchronize(h->ctx.stream);
-    return e.code;
-  } catch (const std::exception& e) {
-    h->ctx.err = e.what();
-    return RR_ERR_INTERNAL;
-  }
-}
-}  // namespace
 
 extern "C" {
 
