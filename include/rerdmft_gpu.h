@@ -209,6 +209,22 @@ int rr_hartree_exchange_sigma_joint_hessian_vector_ri(rr_handle h, size_t n, con
                                                       const double* two_rdm_l2, size_t n_pairs, const size_t* pairs,
                                                       const double* v, double* w);
 
+/* ---- One-time RI tensor preparation (FP64) -----------------------------------------------------
+ * riThreeCenterTensor's contraction: src/AO_ints/ThreeCenterIntegrals.cpp:219-236.
+ * out (n_aux_kept, n*n) = metric_x^T (n_aux_raw x n_aux_kept)^T . raw (n_aux_raw, n*n); the raw
+ * integrals themselves come from the caller's integral engine. */
+int rr_ri_three_center_tensor(rr_handle h, size_t n, size_t n_aux_raw, const double* raw, const double* metric_x,
+                              size_t n_aux_kept, double* out);
+/* transformRiThreeCenterAoLegs: src/AO_ints/ThreeCenterIntegrals.cpp:191-217.
+ * eri3_out[P] = t^T eri3_in[P] t, t is n_ao x n_out. */
+int rr_transform_ri_three_center_ao_legs(rr_handle h, size_t n_aux, size_t n_ao, const double* eri3_in,
+                                         const double* t, size_t n_out, double* eri3_out);
+/* canonicalOrthogonalize (the auxiliary-metric step of auxMetricOrthogonalization,
+ * src/AO_ints/ThreeCenterIntegrals.cpp:186-189): src/Linear_Algebra/LinearAlgebra.cpp:267-310.
+ * x receives n x n_kept (row-major; the caller provides n*n doubles); n_kept = eigenvalues > threshold.
+ * With nothing dropped x is the symmetric S^-1/2. */
+int rr_canonical_orthogonalize(rr_handle h, size_t n, const double* s, double threshold, double* x, size_t* n_kept);
+
 #ifdef __cplusplus
 }
 #endif

@@ -308,3 +308,35 @@ def sigma_hessian_vector_ri(h, b, occ, hc, xc, pairs, v, fock=None, pair_of=None
                                                       _p(bb), _sz(naux), _p(oc), _p(a), _p(x), _p(ff), _p(pa),
                                                       _p(l1a), _p(l2a), _sz(npairs), _p(pp), _p(vv), _p(w)))
     return w
+
+
+# ---- one-time RI tensor preparation (rr_oracle.cpp) ------------------------------------------------
+def ri_three_center_tensor(raw, x):
+    """riThreeCenterTensor's contraction (src/AO_ints/ThreeCenterIntegrals.cpp:219-236)."""
+    naux_raw = raw.shape[0]
+    n2 = int(np.prod(raw.shape[1:]))
+    n = int(round(np.sqrt(n2)))
+    kept = x.shape[1]
+    r, xx = _f64(raw), _f64(x)
+    out = np.zeros((kept, n2))
+    _check(lib().oracle_ri_three_center_tensor(_sz(n), _sz(naux_raw), _p(r), _p(xx), _sz(kept), _p(out)))
+    return out
+
+
+def transform_ao_legs(eri3, n_ao, t):
+    """transformRiThreeCenterAoLegs (src/AO_ints/ThreeCenterIntegrals.cpp:191-217)."""
+    naux, n_out = eri3.shape[0], t.shape[1]
+    e, tt = _f64(eri3), _f64(t)
+    out = np.zeros((naux, n_out * n_out))
+    _check(lib().oracle_transform_ao_legs(_sz(naux), _sz(n_ao), _p(e), _p(tt), _sz(n_out), _p(out)))
+    return out
+
+
+def canonical_orthogonalize(s, threshold=1e-10):
+    """canonicalOrthogonalize (src/Linear_Algebra/LinearAlgebra.cpp:267-310)."""
+    n = s.shape[0]
+    ss = _f64(s)
+    x = np.zeros(n * n)
+    kept = _sz()
+    _check(lib().oracle_canonical_orthogonalize(_sz(n), _p(ss), ctypes.c_double(threshold), _p(x), ctypes.byref(kept)))
+    return x[: n * kept.value].reshape(n, kept.value)

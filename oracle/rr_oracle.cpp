@@ -805,4 +805,54 @@ int oracle_eig_hermitian(std::size_t n, const double* a_, double* w, double* v_)
   }
 }
 
+// riThreeCenterTensor's contraction (src/AO_ints/ThreeCenterIntegrals.cpp:219-236): out = X^T raw.
+int oracle_ri_three_center_tensor(std::size_t n, std::size_t naux_raw, const double* raw, const double* x,
+                                  std::size_t kept, double* out) {
+  try {
+    const std::size_t n2 = n * n;
+    gemm(true, false, kept, n2, naux_raw, 1.0, x, kept, raw, n2, 0.0, out, n2);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// transformRiThreeCenterAoLegs (src/AO_ints/ThreeCenterIntegrals.cpp:191-217): out_P = t^T B_P t.
+int oracle_transform_ao_legs(std::size_t naux, std::size_t n_ao, const double* in, const double* t, std::size_t n_out,
+                             double* out) {
+  try {
+    std::vector<double> u(n_out * n_ao);
+    for (std::size_t P = 0; P < naux; ++P) {
+      gemm(true, false, n_out, n_ao, n_ao, 1.0, t, n_out, in + P * n_ao * n_ao, n_ao, 0.0, u.data(), n_ao);
+      gemm(false, false, n_out, n_out, n_ao, 1.0, u.data(), n_ao, t, n_out, 0.0, out + P * n_out * n_out, n_out);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// canonicalOrthogonalize (src/Linear_Algebra/LinearAlgebra.cpp:267-310); x is n x n_kept.
+int oracle_canonical_orthogonalize(std::size_t n, const double* s, double threshold, double* x, std::size_t* n_kept) {
+  try {
+    std::vector<double> w, v;
+    eigSym(n, s, w, v);  // v[i*n + j] = component i of eigenvector j
+    std::vector<std::size_t> keep;
+    for (std::size_t i = 0; i < n; ++i)
+      if (w[i] > threshold) keep.push_back(i);
+    const std::size_t ne = keep.size();
+    *n_kept = ne;
+    std::vector<double> xx(n * ne);
+    for (std::size_t j = 0; j < ne; ++j)
+      for (std::size_t i = 0; i < n; ++i) xx[i * ne + j] = v[i * n + keep[j]] / std::sqrt(w[keep[j]]);
+    if (ne == n)
+      gemm(false, true, n, n, n, 1.0, xx.data(), n, v.data(), n, 0.0, x, n);
+    else
+      std::copy(xx.begin(), xx.end(), x);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
 }  // extern "C"

@@ -33,6 +33,7 @@ EXPORTED = [
     "rr_ri_hx_fock_gemm", "rr_ri_sigma_toolkit", "rr_ri_pair_contract", "rr_ri_diag_contract", "rr_ri_accum_u",
     "rr_jk_only_sigma_hessian_vector_ri", "rr_jk_only_sigma_joint_hessian_vector_ri",
     "rr_hartree_exchange_sigma_hessian_vector_ri", "rr_hartree_exchange_sigma_joint_hessian_vector_ri",
+    "rr_ri_three_center_tensor", "rr_transform_ri_three_center_ao_legs", "rr_canonical_orthogonalize",
 ]
 
 
@@ -471,3 +472,35 @@ class Context:
         """hartreeExchangeSigmaJointHessianVectorRi -- HartreeExchangeSigmaHessianVectorRi.cpp:486-539."""
         return self._sigma("rr_hartree_exchange_sigma_joint_hessian_vector_ri", True, h_mo, b, occupations, two_rdm_h,
                            two_rdm_x, pairs, v, fock, pair_of, two_rdm_l1, two_rdm_l2, hx=True)
+
+    # ---- one-time RI tensor preparation (ThreeCenterIntegrals.cpp) ---------------------------------
+    def ri_three_center_tensor(self, raw, metric_x, out=None):
+        """riThreeCenterTensor's contraction -- src/AO_ints/ThreeCenterIntegrals.cpp:219-236:
+        metric_x^T . raw, raw (n_aux_raw, n*n) -> (n_aux_kept, n*n)."""
+        naux_raw = raw.shape[0]
+        n2 = int(np.prod(raw.shape[1:]))
+        n = int(round(np.sqrt(n2)))
+        kept = metric_x.shape[1]
+        r, x = _f64(raw), _f64(metric_x)
+        o, op = _out(out, (kept, n2), np.float64)
+        self._call("rr_ri_three_center_tensor", _sz(n), _sz(naux_raw), r.ptr, x.ptr, _sz(kept), op)
+        return o
+
+    def transform_ri_three_center_ao_legs(self, eri3, n_ao, t, out=None):
+        """transformRiThreeCenterAoLegs -- src/AO_ints/ThreeCenterIntegrals.cpp:191-217 (t^T B_P t)."""
+        naux = eri3.shape[0]
+        n_out = t.shape[1]
+        e, tt = _f64(eri3), _f64(t)
+        o, op = _out(out, (naux, n_out * n_out), np.float64)
+        self._call("rr_transform_ri_three_center_ao_legs", _sz(naux), _sz(n_ao), e.ptr, tt.ptr, _sz(n_out), op)
+        return o
+
+    def canonical_orthogonalize(self, s, threshold=1e-10):
+        """canonicalOrthogonalize -- src/Linear_Algebra/LinearAlgebra.cpp:267-310. Returns X (n x n_kept)."""
+        n = s.shape[0]
+        ss = _f64(s)
+        x = np.zeros(n * n)
+        kept = _sz()
+        self._call("rr_canonical_orthogonalize", _sz(n), ss.ptr, ctypes.c_double(threshold),
+                   ctypes.c_void_p(x.ctypes.data), ctypes.byref(kept))
+        return x[: n * kept.value].reshape(n, kept.value)
