@@ -225,6 +225,22 @@ int rr_transform_ri_three_center_ao_legs(rr_handle h, size_t n_aux, size_t n_ao,
  * With nothing dropped x is the symmetric S^-1/2. */
 int rr_canonical_orthogonalize(rr_handle h, size_t n, const double* s, double threshold, double* x, size_t* n_kept);
 
+/* Fused functional-stage Fock builds (one call per orbital-optimisation step; the MO tensor stays
+ * on the device). The functional enters through the couplings two_rdm_h/x (and pair terms), which
+ * the caller computes (jkHartreeCoupling, PNOF/pCCD 2-RDM builders).
+ * riJkOnlyFockMatrix / riPnofFockMatrix body: src/Hessian_opt/RiJkOnlyFock.cpp:14-33,
+ * RiPnofFock.cpp:14-30 (C_ukb = v_total c_rkb, buildRiMoEri, h_mo = C^H h C, riHxFockGemm).
+ * h_rkb n_rkb x n_rkb, v_total (2nl+2ns) x n_rkb, c_rkb n_rkb x n_mo (complex); fock n_mo x n_mo. */
+int rr_ri_mo_fock_rkb(rr_handle h, int slot_ll, int slot_ss, size_t nl, size_t ns, const double* h_rkb,
+                      const double* v_total, size_t n_rkb, const double* c_rkb, size_t n_mo, const double* occupations,
+                      const double* two_rdm_h, const double* two_rdm_x, const size_t* pair_of, const double* two_rdm_l1,
+                      const double* two_rdm_l2, double* fock);
+/* riJkOnlyFockMatrixNonRel / riPnofFockMatrixNonRel body: src/Hessian_opt/RiNonRelFullOptFock.cpp:53-98.
+ * h_core_ao n x n, c_current n x 2n (real); couplings and fock 2n x 2n. */
+int rr_ri_mo_fock_nonrel(rr_handle h, int slot, size_t n_spatial, const double* h_core_ao, const double* c_current,
+                         const double* occupations, const double* two_rdm_h, const double* two_rdm_x,
+                         const size_t* pair_of, const double* two_rdm_l1, const double* two_rdm_l2, double* fock);
+
 #ifdef __cplusplus
 }
 #endif

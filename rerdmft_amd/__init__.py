@@ -34,6 +34,7 @@ EXPORTED = [
     "rr_jk_only_sigma_hessian_vector_ri", "rr_jk_only_sigma_joint_hessian_vector_ri",
     "rr_hartree_exchange_sigma_hessian_vector_ri", "rr_hartree_exchange_sigma_joint_hessian_vector_ri",
     "rr_ri_three_center_tensor", "rr_transform_ri_three_center_ao_legs", "rr_canonical_orthogonalize",
+    "rr_ri_mo_fock_rkb", "rr_ri_mo_fock_nonrel",
 ]
 
 
@@ -504,3 +505,31 @@ class Context:
         self._call("rr_canonical_orthogonalize", _sz(n), ss.ptr, ctypes.c_double(threshold),
                    ctypes.c_void_p(x.ctypes.data), ctypes.byref(kept))
         return x[: n * kept.value].reshape(n, kept.value)
+
+    # ---- fused functional-stage Fock builds (RiJkOnlyFock / RiPnofFock / RiNonRelFullOptFock) -------
+    def ri_mo_fock_rkb(self, slot_ll, slot_ss, nl, ns, h_rkb, v_total, c_rkb, occupations, two_rdm_h, two_rdm_x,
+                       pair_of=None, two_rdm_l1=None, two_rdm_l2=None, out=None):
+        """riJkOnlyFockMatrix / riPnofFockMatrix body -- src/Hessian_opt/RiJkOnlyFock.cpp:14-33."""
+        n_rkb, n_mo = c_rkb.shape
+        hh, vv, cc = _c128(h_rkb), _c128(v_total), _c128(c_rkb)
+        oc, h2, x2 = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
+        l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+        o, op = _out(out, (n_mo, n_mo), np.complex128)
+        self._call("rr_ri_mo_fock_rkb", ctypes.c_int(slot_ll), ctypes.c_int(slot_ss), _sz(nl), _sz(ns), hh.ptr, vv.ptr,
+                   _sz(n_rkb), cc.ptr, _sz(n_mo), oc.ptr, h2.ptr, x2.ptr,
+                   ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr, op)
+        return o
+
+    def ri_mo_fock_nonrel(self, slot, h_core_ao, c_current, occupations, two_rdm_h, two_rdm_x, pair_of=None,
+                          two_rdm_l1=None, two_rdm_l2=None, out=None):
+        """riJkOnlyFockMatrixNonRel / riPnofFockMatrixNonRel body -- RiNonRelFullOptFock.cpp:53-98."""
+        ns = h_core_ao.shape[0]
+        hh, cc = _f64(h_core_ao), _f64(c_current)
+        oc, h2, x2 = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
+        l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+        o, op = _out(out, (2 * ns, 2 * ns), np.float64)
+        self._call("rr_ri_mo_fock_nonrel", ctypes.c_int(slot), _sz(ns), hh.ptr, cc.ptr, oc.ptr, h2.ptr, x2.ptr,
+                   ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr, op)
+        return o

@@ -127,6 +127,12 @@ void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, 
 void eig_device(Context& c, bool cplx, size_t n, double* a, double* w);
 Tensor& get_slot(Context& c, int slot);
 void check_prec(int prec);
+// FP64 MO tensors into device memory: buildRiMoEri (complex, (n_aux, nt^2)) and the embedded
+// NON_REL spin-orbital tensor (real, (n_aux, (2n)^2)).
+void mo_tensor_rkb(Context& c, int slot_ll, int slot_ss, size_t nl, size_t ns, const double* dC_ukb, size_t nt,
+                   double* dout);
+void mo_tensor_nonrel_embedded(Context& c, int slot, size_t n, const double* dC_spatial, double* dout);
+size_t slot_naux(Context& c, int slot);
 
 // Output helper: device pointer if the caller's buffer is device memory, else a workspace buffer
 // that is copied back by finish_out().

@@ -1693,6 +1693,36 @@ __global__ void k_split_block(const double* p, size_t n, size_t s, size_t u, dou
   }
 }
 
+// FP64 MO tensors written straight into device memory, for the fused functional Fock builds
+// (riJkOnlyFockMatrix / riPnofFockMatrix and their NON_REL counterparts).
+void mo_tensor_rkb(Context& c, int slot_ll, int slot_ss, size_t nl, size_t ns, const double* dC_ukb, size_t nt,
+                   double* dout) {
+  Tensor& tl = get_slot(c, slot_ll);
+  RR_REQUIRE(tl.n == nl, RR_ERR_DIM, "buildRiMoEri: c_ukb row count does not match 2*nLarge+2*nSmall");
+  RR_REQUIRE(tl.f64, RR_ERR_ARG, "tensor slot has no copy in the requested precision");
+  std::vector<Flavor> fl = {{&tl, 0, nl}, {&tl, nl, nl}};
+  if (ns > 0) {
+    Tensor& ts = get_slot(c, slot_ss);
+    RR_REQUIRE(ts.n == ns && ts.naux == tl.naux, RR_ERR_DIM, "buildRiMoEri: c_ukb row count does not match 2*nLarge+2*nSmall");
+    RR_REQUIRE(ts.f64, RR_ERR_ARG, "tensor slot has no copy in the requested precision");
+    fl.push_back({&ts, 2 * nl, ns});
+    fl.push_back({&ts, 2 * nl + ns, ns});
+  }
+  mo_cplx_slot<double, double>(c, fl, P64, dC_ukb, nt, 2 * nl + 2 * ns, dout);
+}
+
+void mo_tensor_nonrel_embedded(Context& c, int slot, size_t n, const double* dC_spatial, double* dout) {
+  Tensor& t = get_slot(c, slot);
+  RR_REQUIRE(t.n == n, RR_ERR_DIM, "buildRiNonRelSpinMoEri: eri3_L column count does not match n_spatial*n_spatial");
+  RR_REQUIRE(t.f64, RR_ERR_ARG, "tensor slot has no copy in the requested precision");
+  double* mo = static_cast<double*>(wsbuf(c, WS_MO_OUT, std::max<size_t>(1, t.naux * n * n) * sizeof(double)));
+  mo_real_slot<double, double>(c, t, P64, dC_spatial, n, mo);
+  hipLaunchKernelGGL(k_embedded_b, dim3(grid_for(t.naux * 4 * n * n)), dim3(256), 0, c.stream, mo, n, t.naux, dout);
+  RR_HIP(hipGetLastError());
+}
+
+size_t slot_naux(Context& c, int slot) { return get_slot(c, slot).naux; }
+
 }  // namespace rr
 
 // ================================================================================================

@@ -24,6 +24,7 @@ enum SigWs {
   WS_SG_H, WS_SG_U1, WS_SG_U2, WS_SG_U3, WS_SG_PHI, WS_SG_PSI, WS_SG_TMP1, WS_SG_TMP2, WS_SG_TMP3, WS_SG_SMALL,
   WS_SG_OUT, WS_SG_TARGET, WS_SG_DPAD, WS_SG_WMP, WS_SG_Y, WS_SG_W, WS_SG_CMAT, WS_SG_PAIR, WS_SG_X2, WS_SG_H2T,
   WS_SG_FOCK, WS_SG_OCC, WS_SG_HC, WS_SG_XC, WS_SG_FOCKOUT, WS_SG_TARGET2, WS_SG_KIN, WS_SG_FOCKIN,
+  WS_SG_CUKB, WS_SG_MO, WS_SG_HMO, WS_SG_HTMP, WS_SG_CSP, WS_SG_V2, WS_SG_C2, WS_SG_H2IN,
   WS_SG_OPF = WS_SIGMA_BASE + 500, WS_SG_OPG = WS_SIGMA_BASE + 510
 };
 
@@ -325,6 +326,14 @@ __global__ void k_psi_hx_small(const double* KtF, const double* KCF, const doubl
     load_el<CPLX>(KH, d * n + c, r3, i3);
     load_el<CPLX>(Psi, e, tr, ti);
     store_el<CPLX>(Psi, e, tr + r0 + r1 - occ[c] * r2 - occ[d] * r3, ti + i0 + i1 - occ[c] * i2 - occ[d] * i3);
+  }
+}
+
+// h_mo(p,q) = 0 unless p and q are in the same spin block of size ns (buildSpinOneElectron)
+__global__ void k_spin_blocks(double* h, size_t ns) {
+  const size_t n = 2 * ns;
+  RR_GRID_LOOP(e, n * n) {
+    if ((e / n) / ns != (e % n) / ns) h[e] = 0.0;
   }
 }
 
@@ -1337,6 +1346,102 @@ int rr_hartree_exchange_sigma_joint_hessian_vector_ri(rr_handle h, size_t n, con
                                                       const double* v, double* w) {
   return sigma_entry(h, true, true, "hartreeExchangeSigmaJointHessianVectorRi", n, h_mo, b, n_aux, occupations,
                      two_rdm_h, two_rdm_x, fock, pair_of, two_rdm_l1, two_rdm_l2, n_pairs, pairs, v, w);
+}
+
+// riJkOnlyFockMatrix / riPnofFockMatrix / riPccdFockMatrix body (RiJkOnlyFock.cpp:14-33,
+// RiPnofFock.cpp:14-30, RiPccdFock.cpp:14-31): C_ukb = V C, B^MO = buildRiMoEri(C_ukb) kept on the
+// device, h_mo = C^H h C, then the generalized Fock GEMM; the functional enters only through the
+// couplings (and pair terms) the caller passes.
+int rr_ri_mo_fock_rkb(rr_handle h, int slot_ll, int slot_ss, size_t nl, size_t ns, const double* h_rkb,
+                      const double* v_total, size_t n_rkb, const double* c_rkb, size_t n_mo, const double* occupations,
+                      const double* two_rdm_h, const double* two_rdm_x, const size_t* pair_of, const double* two_rdm_l1,
+                      const double* two_rdm_l2, double* fock) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(h_rkb && v_total && c_rkb && occupations && two_rdm_h && two_rdm_x && fock, RR_ERR_ARG,
+               "riJkOnlyFockMatrix: null argument");
+    RR_REQUIRE(!pair_of || (two_rdm_l1 && two_rdm_l2), RR_ERR_ARG,
+               "riHxFockGemm: pair_of/two_rdm_l1/two_rdm_l2 dimensions inconsistent with h");
+    const size_t nu = 2 * nl + 2 * ns;
+    const size_t naux = rr::slot_naux(c, slot_ll);
+    const double* dh = dev_in(c, rr::WS_SG_H2IN, h_rkb, 2 * n_rkb * n_rkb);
+    const double* dv = dev_in(c, rr::WS_SG_V2, v_total, 2 * nu * n_rkb);
+    const double* dc = dev_in(c, rr::WS_SG_C2, c_rkb, 2 * n_rkb * n_mo);
+    double* cukb = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_CUKB, std::max<size_t>(1, 2 * nu * n_mo) * sizeof(double)));
+    rr::dgemm_small(c, true, 'N', 'N', nu, n_mo, n_rkb, dv, n_rkb, dc, n_mo, cukb, n_mo);  // C_ukb = V C
+    double* mo = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_MO, std::max<size_t>(1, 2 * naux * n_mo * n_mo) * sizeof(double)));
+    rr::mo_tensor_rkb(c, slot_ll, slot_ss, nl, ns, cukb, n_mo, mo);
+    double* tmp = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_HTMP, std::max<size_t>(1, 2 * n_rkb * n_mo) * sizeof(double)));
+    double* hmo = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_HMO, std::max<size_t>(1, 2 * n_mo * n_mo) * sizeof(double)));
+    rr::dgemm_small(c, true, 'N', 'N', n_rkb, n_mo, n_rkb, dh, n_rkb, dc, n_mo, tmp, n_mo);     // h C
+    rr::dgemm_small(c, true, 'C', 'N', n_mo, n_mo, n_rkb, dc, n_mo, tmp, n_mo, hmo, n_mo);      // C^H h C
+    const double* docc = dev_in(c, rr::WS_SG_OCC, occupations, n_mo);
+    const double* dh2 = dev_in(c, rr::WS_SG_HC, two_rdm_h, n_mo * n_mo);
+    const double* dx2 = dev_in(c, rr::WS_SG_XC, two_rdm_x, n_mo * n_mo);
+    const int64_t* dpair = nullptr;
+    const double* dcmat = nullptr;
+    if (pair_of) {
+      std::vector<size_t> pr = read_pair_of(n_mo, pair_of);
+      std::vector<double> l1(n_mo * n_mo), l2(n_mo * n_mo), cm(n_mo * n_mo);
+      RR_HIP(hipMemcpy(l1.data(), two_rdm_l1, l1.size() * sizeof(double), hipMemcpyDefault));
+      RR_HIP(hipMemcpy(l2.data(), two_rdm_l2, l2.size() * sizeof(double), hipMemcpyDefault));
+      for (size_t s = 0; s < n_mo; ++s)
+        for (size_t q = 0; q < n_mo; ++q) cm[s * n_mo + q] = l1[s * n_mo + pr[q]] + l2[s * n_mo + q];
+      std::vector<int64_t> pi(pr.begin(), pr.end());
+      dpair = rr::upload_tab(c, rr::WS_SG_PAIR, pi);
+      dcmat = rr::upload_tab(c, rr::WS_SG_CMAT, cm);
+    }
+    rr::OutBuf of = rr::out_buf(c, rr::WS_SG_FOCKOUT, fock, 2 * n_mo * n_mo);
+    rr::hx_fock<true>(c, n_mo, naux, mo, hmo, docc, dh2, dx2, dpair, dcmat, of.dev);
+    rr::finish_out(c, of);
+  });
+}
+
+// riJkOnlyFockMatrixNonRel / riPnofFockMatrixNonRel / riPccdFockMatrixNonRel body
+// (RiNonRelFullOptFock.cpp:53-98): spatial half of c_current (n x 2n), embedded spin-orbital MO
+// tensor, spin-blocked h_mo = C^T h C (buildSpinOneElectron, :38-49), then the generalized Fock GEMM.
+int rr_ri_mo_fock_nonrel(rr_handle h, int slot, size_t n_spatial, const double* h_core_ao, const double* c_current,
+                         const double* occupations, const double* two_rdm_h, const double* two_rdm_x,
+                         const size_t* pair_of, const double* two_rdm_l1, const double* two_rdm_l2, double* fock) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(h_core_ao && c_current && occupations && two_rdm_h && two_rdm_x && fock, RR_ERR_ARG,
+               "riJkOnlyFockMatrixNonRel: null argument");
+    RR_REQUIRE(!pair_of || (two_rdm_l1 && two_rdm_l2), RR_ERR_ARG,
+               "riHxFockGemm: pair_of/two_rdm_l1/two_rdm_l2 dimensions inconsistent with h");
+    const size_t ns = n_spatial, nt = 2 * ns;
+    const size_t naux = rr::slot_naux(c, slot);
+    const double* dh = dev_in(c, rr::WS_SG_H2IN, h_core_ao, ns * ns);
+    const double* dc = dev_in(c, rr::WS_SG_C2, c_current, ns * nt);
+    double* csp = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_CSP, std::max<size_t>(1, ns * ns) * sizeof(double)));
+    hipLaunchKernelGGL((rr::k_pad<false>), dim3(rr::grid_for(ns * ns)), dim3(256), 0, c.stream, dc, 1, nt, 0, ns, ns, ns,
+                       ns, csp);  // spatialHalf: first n_spatial columns
+    double* mo = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_MO, std::max<size_t>(1, naux * nt * nt) * sizeof(double)));
+    rr::mo_tensor_nonrel_embedded(c, slot, ns, csp, mo);
+    double* tmp = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_HTMP, std::max<size_t>(1, nt * ns) * sizeof(double)));
+    double* hmo = static_cast<double*>(rr::wsbuf(c, rr::WS_SG_HMO, std::max<size_t>(1, nt * nt) * sizeof(double)));
+    rr::dgemm_small(c, false, 'T', 'N', nt, ns, ns, dc, nt, dh, ns, tmp, ns);  // C^T h
+    rr::dgemm_small(c, false, 'N', 'N', nt, nt, ns, tmp, ns, dc, nt, hmo, nt);  // C^T h C
+    hipLaunchKernelGGL(rr::k_spin_blocks, dim3(rr::grid_for(nt * nt)), dim3(256), 0, c.stream, hmo, ns);
+    RR_HIP(hipGetLastError());
+    const double* docc = dev_in(c, rr::WS_SG_OCC, occupations, nt);
+    const double* dh2 = dev_in(c, rr::WS_SG_HC, two_rdm_h, nt * nt);
+    const double* dx2 = dev_in(c, rr::WS_SG_XC, two_rdm_x, nt * nt);
+    const int64_t* dpair = nullptr;
+    const double* dcmat = nullptr;
+    if (pair_of) {
+      std::vector<size_t> pr = read_pair_of(nt, pair_of);
+      std::vector<double> l1(nt * nt), l2(nt * nt), cm(nt * nt);
+      RR_HIP(hipMemcpy(l1.data(), two_rdm_l1, l1.size() * sizeof(double), hipMemcpyDefault));
+      RR_HIP(hipMemcpy(l2.data(), two_rdm_l2, l2.size() * sizeof(double), hipMemcpyDefault));
+      for (size_t s = 0; s < nt; ++s)
+        for (size_t q = 0; q < nt; ++q) cm[s * nt + q] = l1[s * nt + pr[q]] + l2[s * nt + q];
+      std::vector<int64_t> pi(pr.begin(), pr.end());
+      dpair = rr::upload_tab(c, rr::WS_SG_PAIR, pi);
+      dcmat = rr::upload_tab(c, rr::WS_SG_CMAT, cm);
+    }
+    rr::OutBuf of = rr::out_buf(c, rr::WS_SG_FOCKOUT, fock, nt * nt);
+    rr::hx_fock<false>(c, nt, naux, mo, hmo, docc, dh2, dx2, dpair, dcmat, of.dev);
+    rr::finish_out(c, of);
+  });
 }
 
 }  // extern "C"

@@ -156,3 +156,50 @@ def test_sigma_rejects_bad_pairs(ctx):
     g, b, h, occ, h2, x2 = case(301, 5, 6, False)
     with pytest.raises(rr.GpuError, match="pair index out of range"):
         ctx.jk_only_sigma_hessian_vector_ri(h, b, occ, h2, x2, [(7, 1)], np.ones(1))
+
+
+def test_fused_functional_fock_rkb(ctx):
+    """riJkOnlyFockMatrix / riPnofFockMatrix body on the device vs the oracle's pieces composed."""
+    from rerdmft_amd import synthetic as syn
+    g = np.random.default_rng(400)
+    nl, ns, naux, n_rkb = 9, 7, 40, 20
+    nu = 2 * nl + 2 * ns
+    bl = syn.ri_tensor(naux, nl, 0.5, 5.0, 401)
+    bs = syn.ri_tensor(naux, ns, 0.5, 5.0, 402)
+    ctx.upload(10, bl)
+    ctx.upload(11, bs)
+    v = g.standard_normal((nu, n_rkb)) + 1j * g.standard_normal((nu, n_rkb))
+    c = g.standard_normal((n_rkb, 12)) + 1j * g.standard_normal((n_rkb, 12))
+    hr = g.standard_normal((n_rkb, n_rkb)) + 1j * g.standard_normal((n_rkb, n_rkb))
+    hr = 0.5 * (hr + hr.conj().T)
+    occ = g.uniform(0, 1, 12)
+    h2, x2 = g.standard_normal((12, 12)), g.standard_normal((12, 12))
+    pr, l1, l2 = pairing(12), g.standard_normal((12, 12)), g.standard_normal((12, 12))
+    for args in ((), (pr, l1, l2)):
+        f = ctx.ri_mo_fock_rkb(10, 11, nl, ns, hr, v, c, occ, h2, x2, *args)
+        bmo = oracle.build_ri_mo_eri(nl, ns, bl.reshape(naux, -1), bs.reshape(naux, -1), v @ c)
+        fr = oracle.ri_hx_fock_gemm(bmo, c.conj().T @ hr @ c, occ, h2, x2, *args)
+        assert rel(f, fr) < TOL
+
+
+def test_fused_functional_fock_nonrel(ctx):
+    """riJkOnlyFockMatrixNonRel body (spatial half, embedded spin tensor, spin-blocked h_mo)."""
+    from rerdmft_amd import synthetic as syn
+    g = np.random.default_rng(410)
+    ns, naux = 11, 45
+    b = syn.ri_tensor(naux, ns, 0.5, 5.0, 411)
+    ctx.upload(12, b)
+    c = g.standard_normal((ns, 2 * ns))
+    hc = syn.symmetric(ns, 412)
+    occ = g.uniform(0, 1, 2 * ns)
+    h2, x2 = g.standard_normal((2 * ns, 2 * ns)), g.standard_normal((2 * ns, 2 * ns))
+    f = ctx.ri_mo_fock_nonrel(12, hc, c, occ, h2, x2)
+    bmo = oracle.build_ri_nonrel_spin_mo_eri(b.reshape(naux, -1), c[:, :ns]).reshape(naux, ns, ns)
+    emb = np.zeros((naux, 2 * ns, 2 * ns))
+    emb[:, :ns, :ns] = bmo
+    emb[:, ns:, ns:] = bmo
+    hmo = c.T @ hc @ c
+    hmo[:ns, ns:] = 0.0
+    hmo[ns:, :ns] = 0.0
+    fr = oracle.ri_hx_fock_gemm(emb, hmo, occ, h2, x2)
+    assert rel(f, fr) < TOL
