@@ -241,6 +241,22 @@ int rr_ri_mo_fock_nonrel(rr_handle h, int slot, size_t n_spatial, const double* 
                          const double* occupations, const double* two_rdm_h, const double* two_rdm_x,
                          const size_t* pair_of, const double* two_rdm_l1, const double* two_rdm_l2, double* fock);
 
+/* ---- 4c RKB Cholesky vectors (RkbCholesky, src/C4_DHF/RkbCholesky.h) ---------------------------
+ * A set holds n_chol vectors, each with components large (real nl x nl), small[0], small[1],
+ * small_cross[0], small_cross[1] (complex nl x nl); arrays are (n_chol, nl*nl) row-major. */
+int rr_rkb_cholesky_upload(rr_handle h, int set, size_t n_large, size_t n_chol, const double* large,
+                           const double* small0, const double* small1, const double* cross_fwd,
+                           const double* cross_bwd);
+int rr_rkb_cholesky_release(rr_handle h, int set);
+/* rkbFockMatrix(h_rkb, RkbCholesky, P): src/C4_DHF/RkbCholesky.cpp:352-448 (n = 4*nl). */
+int rr_rkb_fock_cholesky(rr_handle h, int set, size_t n, const double* h_rkb, const double* p, double* fock);
+/* rkbCholeskyToMo: src/C4_DHF/RkbCholesky.cpp:450-523. c_mo is (4 nl) x n_mo complex. With
+ * n_negative = 0, out receives n_chol vectors of (n_mo x n_mo); otherwise the recompressed vectors
+ * (RkbCholesky.cpp:244-283, eigenvalue cut threshold / n_chol) embedded in n_mo x n_mo. capacity =
+ * the number of vectors out can hold; n_out = the number written. */
+int rr_rkb_cholesky_to_mo(rr_handle h, int set, const double* c_mo, size_t n_mo, size_t n_negative, double threshold,
+                          double* out, size_t capacity, size_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -340,3 +340,36 @@ def canonical_orthogonalize(s, threshold=1e-10):
     kept = _sz()
     _check(lib().oracle_canonical_orthogonalize(_sz(n), _p(ss), ctypes.c_double(threshold), _p(x), ctypes.byref(kept)))
     return x[: n * kept.value].reshape(n, kept.value)
+
+
+# ---- 4c RKB Cholesky (rr_oracle.cpp) ---------------------------------------------------------------
+def _rkb_args(large, small0, small1, cross_f, cross_b):
+    arrs = [_f64(large), _c128(small0), _c128(small1), _c128(cross_f), _c128(cross_b)]
+    return arrs, [_p(a) for a in arrs]
+
+
+def rkb_fock_cholesky(large, small0, small1, cross_f, cross_b, h, p):
+    """rkbFockMatrix(h_rkb, RkbCholesky, P) -- src/C4_DHF/RkbCholesky.cpp:352-448."""
+    nchol = large.shape[0]
+    nl = int(round(np.sqrt(np.prod(large.shape[1:]))))
+    keep, ptrs = _rkb_args(large, small0, small1, cross_f, cross_b)
+    hh, pp = _c128(h), _c128(p)
+    f = np.zeros((4 * nl, 4 * nl), dtype=np.complex128)
+    _check(lib().oracle_rkb_fock_cholesky(_sz(nl), _sz(nchol), *ptrs, _p(hh), _p(pp), _p(f)))
+    return f
+
+
+def rkb_cholesky_to_mo(large, small0, small1, cross_f, cross_b, c, n_negative=0, threshold=1e-10):
+    """rkbCholeskyToMo -- src/C4_DHF/RkbCholesky.cpp:450-523 (recompression :244-283)."""
+    nchol = large.shape[0]
+    nl = int(round(np.sqrt(np.prod(large.shape[1:]))))
+    keep, ptrs = _rkb_args(large, small0, small1, cross_f, cross_b)
+    cc = _c128(c)
+    nmo = cc.shape[1]
+    m = nmo - n_negative
+    per = m * m if n_negative == 0 else nmo * nmo
+    out = np.zeros((nchol, per), dtype=np.complex128)
+    nout = _sz()
+    _check(lib().oracle_rkb_cholesky_to_mo(_sz(nl), _sz(nchol), *ptrs, _p(cc), _sz(nmo), _sz(n_negative),
+                                           ctypes.c_double(threshold), _p(out), _sz(nchol), ctypes.byref(nout)))
+    return out[: nout.value]

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <functional>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
@@ -849,6 +850,151 @@ int oracle_canonical_orthogonalize(std::size_t n, const double* s, double thresh
       gemm(false, true, n, n, n, 1.0, xx.data(), n, v.data(), n, 0.0, x, n);
     else
       std::copy(xx.begin(), xx.end(), x);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// ---- 4c RKB Cholesky (src/C4_DHF/RkbCholesky.cpp) ---------------------------------------------
+// Vector components per L (nl x nl row-major): large (real), small0, small1, cross_fwd, cross_bwd.
+struct RkbVecs {
+  std::size_t nl, nchol;
+  const double* large;
+  const C* sm[2];
+  const C* cross[2];
+  C b(int f, std::size_t L, std::size_t r, std::size_t c) const {  // flavor block B^f_L(r,c)
+    const std::size_t e = L * nl * nl + r * nl + c;
+    return f < 2 ? C(large[e], 0.0) : sm[f - 2][e];
+  }
+};
+
+// rkbFockMatrix(h_rkb, RkbCholesky, P): RkbCholesky.cpp:352-448.
+int oracle_rkb_fock_cholesky(std::size_t nl, std::size_t nchol, const double* large, const double* small0,
+                             const double* small1, const double* cross_f, const double* cross_b, const double* h_,
+                             const double* p_, double* f_) {
+  try {
+    const RkbVecs v{nl, nchol, large, {reinterpret_cast<const C*>(small0), reinterpret_cast<const C*>(small1)},
+                    {reinterpret_cast<const C*>(cross_f), reinterpret_cast<const C*>(cross_b)}};
+    const std::size_t N = 4 * nl;
+    const C* p = reinterpret_cast<const C*>(p_);
+    const C* h = reinterpret_cast<const C*>(h_);
+    C* f = reinterpret_cast<C*>(f_);
+    auto P = [&](int bf, int bg, std::size_t a, std::size_t b) { return p[(bf * nl + a) * N + bg * nl + b]; };
+    std::vector<C> jm(4 * nl * nl, C{}), jx(2 * nl * nl, C{}), km(16 * nl * nl, C{});
+    std::vector<C> tmp(nl * nl);
+    for (std::size_t L = 0; L < nchol; ++L) {
+      C t{};
+      for (int fl = 0; fl < 4; ++fl)
+        for (std::size_t q = 0; q < nl; ++q)
+          for (std::size_t s = 0; s < nl; ++s) t += v.b(fl, L, q, s) * P(fl, fl, s, q);
+      for (std::size_t q = 0; q < nl; ++q)
+        for (std::size_t s = 0; s < nl; ++s) {
+          t += v.cross[0][L * nl * nl + q * nl + s] * P(3, 2, s, q);
+          t += v.cross[1][L * nl * nl + q * nl + s] * P(2, 3, s, q);
+        }
+      for (int fl = 0; fl < 4; ++fl)
+        for (std::size_t e = 0; e < nl * nl; ++e) jm[fl * nl * nl + e] += t * v.b(fl, L, e / nl, e % nl);
+      for (int x = 0; x < 2; ++x)
+        for (std::size_t e = 0; e < nl * nl; ++e) jx[x * nl * nl + e] += t * v.cross[x][L * nl * nl + e];
+      for (int fa = 0; fa < 4; ++fa)
+        for (int fb = 0; fb < 4; ++fb) {
+          for (std::size_t a = 0; a < nl; ++a)  // tmp = P_fg B^g
+            for (std::size_t b = 0; b < nl; ++b) {
+              C acc{};
+              for (std::size_t k = 0; k < nl; ++k) acc += P(fa, fb, a, k) * v.b(fb, L, k, b);
+              tmp[a * nl + b] = acc;
+            }
+          for (std::size_t a = 0; a < nl; ++a)  // K_fg += B^f tmp
+            for (std::size_t b = 0; b < nl; ++b) {
+              C acc{};
+              for (std::size_t k = 0; k < nl; ++k) acc += v.b(fa, L, a, k) * tmp[k * nl + b];
+              km[(fa * 4 + fb) * nl * nl + a * nl + b] += acc;
+            }
+        }
+    }
+    for (std::size_t i = 0; i < N * N; ++i) f[i] = h[i];
+    for (int fa = 0; fa < 4; ++fa)
+      for (std::size_t a = 0; a < nl; ++a)
+        for (std::size_t b = 0; b < nl; ++b) {
+          f[(fa * nl + a) * N + fa * nl + b] += jm[fa * nl * nl + a * nl + b];
+          for (int fb = 0; fb < 4; ++fb) f[(fa * nl + a) * N + fb * nl + b] -= km[(fa * 4 + fb) * nl * nl + a * nl + b];
+        }
+    for (std::size_t a = 0; a < nl; ++a)
+      for (std::size_t b = 0; b < nl; ++b) {
+        f[(2 * nl + a) * N + 3 * nl + b] += jx[a * nl + b];
+        f[(3 * nl + a) * N + 2 * nl + b] += jx[nl * nl + a * nl + b];
+      }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// rkbCholeskyToMo: RkbCholesky.cpp:450-523 (recompression :244-283 when n_negative > 0).
+int oracle_rkb_cholesky_to_mo(std::size_t nl, std::size_t nchol, const double* large, const double* small0,
+                              const double* small1, const double* cross_f, const double* cross_b, const double* c_,
+                              std::size_t nmo, std::size_t nn, double threshold, double* out_, std::size_t capacity,
+                              std::size_t* n_out) {
+  try {
+    const RkbVecs v{nl, nchol, large, {reinterpret_cast<const C*>(small0), reinterpret_cast<const C*>(small1)},
+                    {reinterpret_cast<const C*>(cross_f), reinterpret_cast<const C*>(cross_b)}};
+    const C* cm = reinterpret_cast<const C*>(c_);
+    C* out = reinterpret_cast<C*>(out_);
+    const std::size_t m = nmo - nn;
+    auto Cf = [&](int f, std::size_t a, std::size_t p) { return cm[(f * nl + a) * nmo + nn + p]; };
+    std::vector<C> A(nchol * m * m);
+    std::vector<C> bp(m * m), tmp(nl * m);
+    auto term = [&](int fb, int fk, std::function<C(std::size_t, std::size_t)> B) {
+      for (std::size_t a = 0; a < nl; ++a)
+        for (std::size_t q = 0; q < m; ++q) {
+          C acc{};
+          for (std::size_t k = 0; k < nl; ++k) acc += B(a, k) * Cf(fk, k, q);
+          tmp[a * m + q] = acc;
+        }
+      for (std::size_t p = 0; p < m; ++p)
+        for (std::size_t q = 0; q < m; ++q) {
+          C acc{};
+          for (std::size_t a = 0; a < nl; ++a) acc += std::conj(Cf(fb, a, p)) * tmp[a * m + q];
+          bp[p * m + q] += acc;
+        }
+    };
+    for (std::size_t L = 0; L < nchol; ++L) {
+      std::fill(bp.begin(), bp.end(), C{});
+      for (int f = 0; f < 4; ++f) term(f, f, [&](std::size_t a, std::size_t k) { return v.b(f, L, a, k); });
+      term(2, 3, [&](std::size_t a, std::size_t k) { return v.cross[0][L * nl * nl + a * nl + k]; });
+      term(3, 2, [&](std::size_t a, std::size_t k) { return v.cross[1][L * nl * nl + a * nl + k]; });
+      for (std::size_t x = 0; x < m; ++x)
+        for (std::size_t y = 0; y < m; ++y) A[L * m * m + x * m + y] = bp[y * m + x];
+    }
+    if (nn == 0) {
+      if (capacity < nchol) throw std::runtime_error("rkbCholeskyToMo: output capacity too small");
+      std::copy(A.begin(), A.end(), out);
+      *n_out = nchol;
+      return 0;
+    }
+    std::vector<C> gram(nchol * nchol);
+    zgemm('N', 'C', nchol, nchol, m * m, A.data(), m * m, A.data(), m * m, C(0.0, 0.0), gram.data(), nchol);
+    std::vector<double> w;
+    std::vector<C> u;
+    eigHerm(nchol, gram.data(), w, u);  // u[l*nchol + k] = component l of eigenvector k
+    const double tau = threshold / static_cast<double>(std::max<std::size_t>(1, nchol));
+    std::vector<std::size_t> keep;
+    for (std::size_t k = 0; k < nchol; ++k)
+      if (w[k] > tau) keep.push_back(k);
+    if (keep.empty()) throw std::runtime_error("recompressMoVectors: no Cholesky direction survives the compression");
+    if (capacity < keep.size()) throw std::runtime_error("rkbCholeskyToMo: output capacity too small");
+    for (std::size_t kk = 0; kk < keep.size(); ++kk) {
+      C* o = out + kk * nmo * nmo;
+      std::fill(o, o + nmo * nmo, C{});
+      for (std::size_t x = 0; x < m; ++x)
+        for (std::size_t y = 0; y < m; ++y) {
+          C acc{};
+          for (std::size_t L = 0; L < nchol; ++L) acc += std::conj(u[L * nchol + keep[kk]]) * A[L * m * m + x * m + y];
+          o[(nn + x) * nmo + nn + y] = acc;
+        }
+    }
+    *n_out = keep.size();
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

@@ -35,6 +35,7 @@ EXPORTED = [
     "rr_hartree_exchange_sigma_hessian_vector_ri", "rr_hartree_exchange_sigma_joint_hessian_vector_ri",
     "rr_ri_three_center_tensor", "rr_transform_ri_three_center_ao_legs", "rr_canonical_orthogonalize",
     "rr_ri_mo_fock_rkb", "rr_ri_mo_fock_nonrel",
+    "rr_rkb_cholesky_upload", "rr_rkb_cholesky_release", "rr_rkb_fock_cholesky", "rr_rkb_cholesky_to_mo",
 ]
 
 
@@ -533,3 +534,35 @@ class Context:
         self._call("rr_ri_mo_fock_nonrel", ctypes.c_int(slot), _sz(ns), hh.ptr, cc.ptr, oc.ptr, h2.ptr, x2.ptr,
                    ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr, op)
         return o
+
+    # ---- 4c RKB Cholesky (RkbCholesky.cpp) -------------------------------------------------------
+    def rkb_cholesky_upload(self, set_id, large, small0, small1, cross_fwd, cross_bwd):
+        """Upload RkbCholesky vectors (large real, the four small components complex), (n_chol, nl, nl)."""
+        nchol = large.shape[0]
+        nl = int(round(np.sqrt(np.prod(large.shape[1:]))))
+        arrs = [_f64(large), _c128(small0), _c128(small1), _c128(cross_fwd), _c128(cross_bwd)]
+        self._call("rr_rkb_cholesky_upload", ctypes.c_int(set_id), _sz(nl), _sz(nchol), *[a.ptr for a in arrs])
+
+    def rkb_cholesky_release(self, set_id):
+        self._call("rr_rkb_cholesky_release", ctypes.c_int(set_id))
+
+    def rkb_fock_cholesky(self, set_id, h_rkb, p, out=None):
+        """rkbFockMatrix(h_rkb, RkbCholesky, P) -- src/C4_DHF/RkbCholesky.cpp:352-448."""
+        n = h_rkb.shape[0]
+        hh, pp = _c128(h_rkb), _c128(p)
+        o, op = _out(out, (n, n), np.complex128)
+        self._call("rr_rkb_fock_cholesky", ctypes.c_int(set_id), _sz(n), hh.ptr, pp.ptr, op)
+        return o
+
+    def rkb_cholesky_to_mo(self, set_id, c_mo, n_chol, n_negative=0, threshold=1e-10):
+        """rkbCholeskyToMo -- src/C4_DHF/RkbCholesky.cpp:450-523. Returns (n_out, m*m) for n_negative = 0,
+        else the recompressed (n_out, n_mo*n_mo)."""
+        cc = _c128(c_mo)
+        nmo = c_mo.shape[1]
+        m = nmo - n_negative
+        per = m * m if n_negative == 0 else nmo * nmo
+        out = np.zeros((n_chol, per), dtype=np.complex128)
+        nout = _sz()
+        self._call("rr_rkb_cholesky_to_mo", ctypes.c_int(set_id), cc.ptr, _sz(nmo), _sz(n_negative),
+                   ctypes.c_double(threshold), ctypes.c_void_p(out.ctypes.data), _sz(n_chol), ctypes.byref(nout))
+        return out[: nout.value]

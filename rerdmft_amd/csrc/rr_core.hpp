@@ -35,6 +35,17 @@ struct Tensor {
 
 constexpr int kSlots = 16;
 
+// A set of 4c RKB Cholesky vectors (RkbCholesky, src/C4_DHF/RkbCholesky.h): per vector L the five
+// nl x nl components large (promoted to complex), small[0], small[1], small_cross[0],
+// small_cross[1], complex interleaved, rows padded to ld = roundup(nl, 8). `nat` holds them as
+// given ([comp][L][row][ld]); `tr` holds the first three transposed.
+constexpr int kRkbSets = 4;
+struct RkbChol {
+  size_t nl = 0, nchol = 0, ld = 0;
+  double* nat = nullptr;
+  double* tr = nullptr;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -48,6 +59,7 @@ struct Context {
   int last_ntiles = 0;  // tiles per batch of the most recent GEMM launch
   std::map<std::vector<long long>, std::pair<int, int>> tuned;  // shape key -> (generic cfg, LDS variant)
   float last_ms[8] = {0};
+  RkbChol rkb[kRkbSets];
 };
 
 struct Error {

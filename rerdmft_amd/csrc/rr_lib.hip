@@ -1765,6 +1765,10 @@ int rr_destroy(rr_handle h) {
     if (t.f32) (void)hipFree(t.f32);
     if (t.b16) (void)hipFree(t.b16);
   }
+  for (auto& r : h->ctx.rkb) {
+    if (r.nat) (void)hipFree(r.nat);
+    if (r.tr) (void)hipFree(r.tr);
+  }
   for (auto& kv : h->ctx.ws) (void)hipFree(kv.second.first);
   if (h->ctx.blas) rocblas_destroy_handle(h->ctx.blas);
   (void)hipStreamDestroy(h->ctx.stream);
