@@ -60,6 +60,7 @@ struct Context {
   std::map<std::vector<long long>, std::pair<int, int>> tuned;  // shape key -> (generic cfg, LDS variant)
   float last_ms[8] = {0};
   RkbChol rkb[kRkbSets];
+  void* zpage = nullptr;  // 256 zero bytes (source of out-of-range chunks in direct-to-LDS GEMMs)
 };
 
 struct Error {

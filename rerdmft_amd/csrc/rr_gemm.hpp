@@ -40,6 +40,8 @@ struct GemmArgs {
   // sAk (sBk) logical elements after block 0. kper then counts flat k over kbatch * K.
   int kbatch = 0;
   long long sAk = 0, sBk = 0;
+  // 16+ bytes of zeros in device memory: the source of out-of-range chunks in the direct-to-LDS kernel
+  const void* zpage = nullptr;
 };
 
 
@@ -271,8 +273,9 @@ namespace rr {
 // fragments. The global loads of stage s+1 are in flight while stage s is computed; one barrier
 // per stage. Supports batches, split-K partials, lower-triangle tiles and the second output.
 // ------------------------------------------------------------------------------------------------
-template <typename TI, typename TO, int BM, int BN, int BK, int WGM>
+template <typename TI, typename TO, int BM, int BN, int BK, int WGM, int NBUF = 2>
 __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
+  static_assert(NBUF == 1 || NBUF == 2, "one or two LDS stage buffers");
   using MM = Mma<TI>;
   using acc_t = typename MM::acc_t;
   using frag = typename MM::frag;
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
   constexpr int BV = (BN * VPR + 255) / 256;
   typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
-  __shared__ __attribute__((aligned(16))) TI lds[2][(BM + BN) * LDR];
+  __shared__ __attribute__((aligned(16))) TI lds[NBUF][(BM + BN) * LDR];
 
   const int total = g.ntiles * g.nbatch * g.nsplit;
   int w = static_cast<int>(blockIdx.x);
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
   }
   __syncthreads();
   for (int s = 0; s < nstage; ++s) {
-    const int buf = s & 1;
+    const int buf = NBUF == 2 ? (s & 1) : 0;
     if (s + 1 < nstage) gload(kbeg + static_cast<long long>(s + 1) * BK);
     const TI* la = &lds[buf][0];
     const TI* lb = &lds[buf][BM * LDR];
@@ -400,8 +403,210 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
       for (int j = 0; j < WN; ++j) MM::load(fb[j], lb + (wn0 + 16 * j + r) * LDR + ks + grp * KL);
       MM::mma_tiles(acc, fa, fb);
     }
-    if (s + 1 < nstage) lstore(buf ^ 1);
-    __syncthreads();
+    if constexpr (NBUF == 2) {
+      if (s + 1 < nstage) lstore(buf ^ 1);
+      __syncthreads();
+    } else {
+      // one buffer: every wave has finished reading stage s before it is overwritten
+      __syncthreads();
+      if (s + 1 < nstage) {
+        lstore(0);
+        __syncthreads();
+      }
+    }
+  }
+
+  const long long coff = static_cast<long long>(batch) * g.sCb + static_cast<long long>(split) * g.sCz;
+  TO* Cb = static_cast<TO*>(g.C) + coff;
+  TO* Cb2 = g.C2 ? static_cast<TO*>(g.C2) + coff : nullptr;
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int n = bn0 + wn0 + 16 * j + r;
+      if (n >= g.N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = bm0 + wm0 + 16 * i + MM::out_row(grp, e);
+        if (m >= g.M) continue;
+        int mm = m, nn = n;
+        TO* base = Cb;
+        if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
+        if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        TO* p = base + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
+        double v = g.alpha * static_cast<double>(acc[i][j][e]);
+        if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
+        store_out<TO>(p, v);
+      }
+    }
+}
+
+
+// ------------------------------------------------------------------------------------------------
+// Multi-stage bf16 NT GEMM fed by direct global->LDS loads (global_load_lds_dwordx4).
+//
+// Each stage stages BK = 32 k-elements (64 bytes) of BM A-rows and BN B-rows. A DMA instruction moves
+// 64 lanes x 16 bytes = 16 rows into LDS (lane l -> row l/4, 16-byte slot l%4); the slot holds the
+// global chunk slot ^ ((row/4) % 4), so the 16 rows read by one fragment load fall into distinct
+// banks. Chunks beyond the rows or beyond the k-range are read from a zero page. NST stages are in
+// flight: before computing stage s each wave waits (vmcnt) only for its own loads of stage s, then
+// one barrier publishes the stage and frees the buffer of stage s-1 for stage s+NST-1. The DMA is
+// issued from inline asm so the compiler adds no conservative wait of its own before the LDS reads;
+// the instruction has no register results. Summation order equals the other NT kernels.
+// ------------------------------------------------------------------------------------------------
+__device__ inline unsigned lds_addr(const void* p) {
+  return static_cast<unsigned>(reinterpret_cast<size_t>((const __attribute__((address_space(3))) unsigned char*)p));
+}
+
+__device__ inline void lds_dma16(const void* gptr, unsigned lds_uniform) {
+  const unsigned m0v = __builtin_amdgcn_readfirstlane(lds_uniform);  // wave-uniform by construction
+  unsigned saved;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %2\n\t"
+      "global_load_lds_dwordx4 %1, off\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(saved)
+      : "v"(gptr), "s"(m0v)
+      : "memory");
+}
+
+// s_waitcnt vmcnt(n) for a wave-uniform n (0..15)
+__device__ inline void wait_vmcnt(int n) {
+  switch (n) {
+    case 0: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    case 1: asm volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
+    case 2: asm volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
+    case 3: asm volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
+    case 4: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+    case 5: asm volatile("s_waitcnt vmcnt(5)" ::: "memory"); break;
+    case 6: asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
+    case 7: asm volatile("s_waitcnt vmcnt(7)" ::: "memory"); break;
+    case 8: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+    case 9: asm volatile("s_waitcnt vmcnt(9)" ::: "memory"); break;
+    case 10: asm volatile("s_waitcnt vmcnt(10)" ::: "memory"); break;
+    case 11: asm volatile("s_waitcnt vmcnt(11)" ::: "memory"); break;
+    case 12: asm volatile("s_waitcnt vmcnt(12)" ::: "memory"); break;
+    case 13: asm volatile("s_waitcnt vmcnt(13)" ::: "memory"); break;
+    case 14: asm volatile("s_waitcnt vmcnt(14)" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(15)" ::: "memory"); break;
+  }
+}
+
+template <typename TO, int BM, int BN, int WGM, int NST>
+__global__ __launch_bounds__(256) void gemm_nt_dma_kernel(GemmArgs g) {
+  using MM = Mma<bf16_t>;
+  using acc_t = typename MM::acc_t;
+  using frag = typename MM::frag;
+  constexpr int BK = 32;           // k-elements per stage (one 16x16x32 step)
+  constexpr int ROWB = 64;         // bytes per staged row
+  constexpr int TI = (BM + BN) / 16;  // DMA instructions per stage
+  constexpr int STAGE = (BM + BN) * ROWB;
+  constexpr int MAXJ = (TI + 3) / 4;
+  constexpr int WGN = 4 / WGM;
+  constexpr int WM = BM / (16 * WGM), WN = BN / (16 * WGN);
+  static_assert(BM % 16 == 0 && BN % 16 == 0, "tiles are made of 16-row DMA groups");
+  static_assert(WM * 16 * WGM == BM && WN * 16 * WGN == BN, "tile not divisible over the waves");
+  static_assert(NST >= 2 && (MAXJ * (NST - 2)) <= 15, "wait counts must fit the helper");
+
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[NST * STAGE];
+
+  const int total = g.ntiles * g.nbatch * g.nsplit;
+  int w = static_cast<int>(blockIdx.x);
+  if (g.xcd_remap) w = remap_xcd(w, total);
+  const int tile = w % g.ntiles;
+  const int rest = w / g.ntiles;
+  const int batch = rest % g.nbatch;
+  const int split = rest / g.nbatch;
+  int tm, tn;
+  if (g.tri) {
+    int t = tile;
+    int a = static_cast<int>((sqrtf(8.0f * static_cast<float>(t) + 1.0f) - 1.0f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    while (a * (a + 1) / 2 > t) --a;
+    tm = a;
+    tn = t - a * (a + 1) / 2;
+  } else {
+    tm = tile % g.tiles_m;
+    tn = tile / g.tiles_m;
+  }
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: keeps the stage logic scalar
+  const int r = lane & 15, grp = lane >> 4;
+  const int wm0 = (wave % WGM) * (16 * WM), wn0 = (wave / WGM) * (16 * WN);
+  const int bm0 = tm * BM, bn0 = tn * BN;
+
+  const long long kbeg = static_cast<long long>(split) * g.kper;
+  long long kend = kbeg + g.kper;
+  if (kend > g.K) kend = g.K;
+
+  const bf16_t* Ab = static_cast<const bf16_t*>(g.A) + static_cast<long long>(batch) * g.sAb;
+  const bf16_t* Bb = static_cast<const bf16_t*>(g.B) + static_cast<long long>(batch) * g.sBb;
+
+  // DMA sources of this wave: instruction i = wave + 4 j covers staged rows [16 i, 16 i + 16)
+  const int cnt = (TI - wave + 3) / 4;  // wave-uniform
+  const bf16_t* src[MAXJ];
+  int kch[MAXJ];
+#pragma unroll
+  for (int j = 0; j < MAXJ; ++j) {
+    const int i = wave + 4 * j;
+    src[j] = nullptr;
+    kch[j] = 0;
+    if (i < TI) {
+      const int row = 16 * i + (lane >> 2);
+      const int trow = row < BM ? row : row - BM;
+      const int c = (lane & 3) ^ ((trow >> 2) & 3);
+      kch[j] = 8 * c;
+      if (row < BM) {
+        const int m = bm0 + row;
+        if (m < g.M) src[j] = Ab + static_cast<long long>(m) * g.lda + 8 * c;
+      } else {
+        const int n = bn0 + trow;
+        if (n < g.N) src[j] = Bb + static_cast<long long>(n) * g.ldb + 8 * c;
+      }
+    }
+  }
+  const unsigned lbase = __builtin_amdgcn_readfirstlane(lds_addr(lds));
+  auto issue = [&](int st) {
+    const long long k0 = kbeg + static_cast<long long>(st) * BK;
+    const unsigned sb = lbase + static_cast<unsigned>((st % NST) * STAGE);
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) {
+      if (j < cnt) {
+        const void* gp = (src[j] && k0 + kch[j] < kend) ? static_cast<const void*>(src[j] + k0) : g.zpage;
+        lds_dma16(gp, sb + static_cast<unsigned>((wave + 4 * j) * 1024));
+      }
+    }
+  };
+
+  acc_t acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = acc_t{};
+
+  const int nstage = static_cast<int>((kend - kbeg + BK - 1) / BK);
+  for (int st = 0; st < NST - 1 && st < nstage; ++st) issue(st);
+  for (int st = 0; st < nstage; ++st) {
+    const int ahead = min(NST - 2, nstage - 1 - st);  // own stages issued after st
+    wait_vmcnt(cnt * ahead);
+    __builtin_amdgcn_s_barrier();
+    if (st + NST - 1 < nstage) issue(st + NST - 1);
+    const unsigned char* base = lds + (st % NST) * STAGE;
+    frag fa[WM], fb[WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      const int row = wm0 + 16 * i + r;
+      MM::load(fa[i], reinterpret_cast<const bf16_t*>(base + row * ROWB + ((grp ^ ((row >> 2) & 3)) << 4)));
+    }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int row = wn0 + 16 * j + r;
+      MM::load(fb[j], reinterpret_cast<const bf16_t*>(base + (BM + row) * ROWB + ((grp ^ ((row >> 2) & 3)) << 4)));
+    }
+    MM::mma_tiles(acc, fa, fb);
   }
 
   const long long coff = static_cast<long long>(batch) * g.sCb + static_cast<long long>(split) * g.sCz;

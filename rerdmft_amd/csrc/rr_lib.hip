@@ -413,9 +413,14 @@ void launch_gemm_cfg(int cfg, const GemmArgs& g, int grid, hipStream_t s) {
 // LDS-staged kernel variants: (BM, BN, waves along M)
 struct LdsCfg {
   int bm, bn, wgm;
+  int bf16_only;  // single-buffered, higher-occupancy variants built for bf16 operands only
 };
-static const LdsCfg kLds[] = {{128, 128, 2}, {128, 208, 4}, {256, 128, 2}, {64, 208, 4},
-                              {64, 224, 2},  {256, 208, 4}, {128, 224, 2}};
+static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 0}, {64, 208, 4, 0},
+                              {64, 224, 2, 0},  {256, 208, 4, 0}, {128, 224, 2, 0}, {128, 208, 4, 1},
+                              {128, 208, 4, 1}, {256, 208, 4, 1}, {128, 128, 2, 1}, {64, 208, 4, 1},
+                              // multi-stage direct-to-LDS (bf16): 12-16
+                              {128, 208, 4, 1}, {128, 208, 4, 1}, {64, 208, 4, 1}, {128, 128, 2, 1},
+                              {256, 128, 2, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 template <typename TI, typename TO>
@@ -435,6 +440,21 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 2: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 256, 128, (B16 ? BK2 : SUB), 2>), dim3(grid), dim3(256), 0, s, g); return;
           case 5: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 256, 208, SUB, 4>), dim3(grid), dim3(256), 0, s, g); return;
           case 6: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 128, 224, (B16 ? BK2 : SUB), 2>), dim3(grid), dim3(256), 0, s, g); return;
+          default: break;
+        }
+      }
+      if constexpr (B16) {
+        switch (v) {
+          case 7: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 128, 208, SUB, 4, 1>), dim3(grid), dim3(256), 0, s, g); return;
+          case 8: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 128, 208, BK2, 4, 1>), dim3(grid), dim3(256), 0, s, g); return;
+          case 9: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 256, 208, SUB, 4, 1>), dim3(grid), dim3(256), 0, s, g); return;
+          case 10: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 128, 128, BK2, 2, 1>), dim3(grid), dim3(256), 0, s, g); return;
+          case 11: hipLaunchKernelGGL((gemm_nt_lds_kernel<TI, TO, 64, 208, BK2, 4, 1>), dim3(grid), dim3(256), 0, s, g); return;
+          case 12: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 128, 208, 4, 3>), dim3(grid), dim3(256), 0, s, g); return;
+          case 13: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 128, 208, 4, 4>), dim3(grid), dim3(256), 0, s, g); return;
+          case 14: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 64, 208, 4, 4>), dim3(grid), dim3(256), 0, s, g); return;
+          case 15: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 128, 128, 2, 4>), dim3(grid), dim3(256), 0, s, g); return;
+          case 16: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 256, 128, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
           default: break;
         }
       }
@@ -462,6 +482,7 @@ void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
   RR_REQUIRE(total < (1ll << 31), RR_ERR_ARG, "GEMM grid too large");
   const int grid = (int)total;
   c.last_ntiles = g.ntiles;
+  g.zpage = c.zpage;
   if (use_lds) {
     if (sp.prec == P64 && sp.out == P64) launch_lds<double, double>(sp.lds, g, grid, c.stream);
     else if (sp.prec == P32 && sp.out == P32) launch_lds<float, float>(sp.lds, g, grid, c.stream);
@@ -504,7 +525,8 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
     if (c.lds_override >= 0 && !sp.cplx && g.kbatch == 0) {
       const LdsCfg& v = kLds[c.lds_override];
       const bool fp64_wide = sp.prec == P64 && (c.lds_override == 2 || c.lds_override == 5 || c.lds_override == 6);
-      if ((!g.tri || v.bm == v.bn) && !fp64_wide) sp.lds = c.lds_override;
+      const bool unavailable = v.bf16_only && sp.prec != P16;
+      if ((!g.tri || v.bm == v.bn) && !fp64_wide && !unavailable) sp.lds = c.lds_override;
     }
     if (c.lds_override == -2) sp.lds = -1;
     run_gemm(c, sp, g);
@@ -532,6 +554,7 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
     for (int v = 0; v < kNumLds; ++v) {
       if (g.tri && kLds[v].bm != kLds[v].bn) continue;
       if (sp.prec == P64 && (v == 2 || v == 5 || v == 6)) continue;  // these exceed the fp64 register budget
+      if (kLds[v].bf16_only && sp.prec != P16) continue;
       cand.push_back({1, v});
     }
   hipEvent_t e0, e1;
@@ -1746,6 +1769,11 @@ int rr_create(int device, rr_handle* out) {
     delete h;
     return RR_ERR_HIP;
   }
+  if (hipMalloc(&h->ctx.zpage, 256) != hipSuccess || hipMemset(h->ctx.zpage, 0, 256) != hipSuccess) {
+    (void)hipStreamDestroy(h->ctx.stream);
+    delete h;
+    return RR_ERR_HIP;
+  }
   if (rocblas_create_handle(&h->ctx.blas) != rocblas_status_success ||
       rocblas_set_stream(h->ctx.blas, h->ctx.stream) != rocblas_status_success) {
     (void)hipStreamDestroy(h->ctx.stream);
@@ -1770,6 +1798,7 @@ int rr_destroy(rr_handle h) {
     if (r.tr) (void)hipFree(r.tr);
   }
   for (auto& kv : h->ctx.ws) (void)hipFree(kv.second.first);
+  if (h->ctx.zpage) (void)hipFree(h->ctx.zpage);
   if (h->ctx.blas) rocblas_destroy_handle(h->ctx.blas);
   (void)hipStreamDestroy(h->ctx.stream);
   delete h;

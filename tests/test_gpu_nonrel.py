@@ -162,3 +162,28 @@ def test_empty_occupied_space(ctx):
     ctx.upload(7, b)
     j, k = ctx.ri_jk_factor(7, np.zeros((6, 6)), np.zeros((6, 0)))
     assert np.all(k == 0.0) and np.all(j == 0.0)
+
+
+@pytest.mark.parametrize("naux,n,r", [(40, 37, 7), (120, 130, 29), (64, 200, 64)])
+def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
+    """Every bf16 GEMM variant (register-staged, LDS-staged single/double buffered, multi-stage
+    direct-to-LDS) sums each output in the same k order: results must be bitwise identical."""
+    b = syn.ri_tensor(naux, n, 1.0, 10.0, 300 + n)
+    g = syn.rng(301 + n)
+    c_old = syn.orthonormal_occ(n, r, 302 + n)
+    q, _ = np.linalg.qr(g.standard_normal((n, n)))
+    c_new = syn.rotate_occ(c_old, q, 0.05, 303 + n)
+    l = g.standard_normal((n, r))
+    ctx.upload(9, b, rr.COPY_BF16)
+    outs = []
+    try:
+        for cfg in [1, 2, 5, 100, 101] + list(range(102, 117)):
+            ctx.set_tile_config(cfg)
+            dj, dk = ctx.ri_jk_delta(9, c_new, c_old, precision=rr.BF16)
+            j, k = ctx.ri_jk_factor(9, None, l, precision=rr.BF16)
+            outs.append((cfg, dk, k))
+    finally:
+        ctx.set_tile_config(-1)
+    for cfg, dk, k in outs[1:]:
+        assert np.array_equal(dk, outs[0][1]), f"incremental K differs for configuration {cfg}"
+        assert np.array_equal(k, outs[0][2]), f"K differs for configuration {cfg}"
