@@ -447,8 +447,8 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
 //
 // Each stage stages BK = 32 k-elements (64 bytes) of BM A-rows and BN B-rows. A DMA instruction moves
 // 64 lanes x 16 bytes = 16 rows into LDS (lane l -> row l/4, 16-byte slot l%4); the slot holds the
-// global chunk slot ^ ((row/4) % 4), so the 16 rows read by one fragment load fall into distinct
-// banks. Chunks beyond the rows or beyond the k-range are read from a zero page. NST stages are in
+// global chunk slot ^ ((row/2) % 4), so each 8-lane phase of a fragment load (8 consecutive rows,
+// one chunk) covers all 32 LDS banks once. Chunks beyond the rows or beyond the k-range are read from a zero page. NST stages are in
 // flight: before computing stage s each wave waits (vmcnt) only for its own loads of stage s, then
 // one barrier publishes the stage and frees the buffer of stage s-1 for stage s+NST-1. The DMA is
 // issued from inline asm so the compiler adds no conservative wait of its own before the LDS reads;
@@ -557,7 +557,7 @@ __global__ __launch_bounds__(256) void gemm_nt_dma_kernel(GemmArgs g) {
     if (i < TI) {
       const int row = 16 * i + (lane >> 2);
       const int trow = row < BM ? row : row - BM;
-      const int c = (lane & 3) ^ ((trow >> 2) & 3);
+      const int c = (lane & 3) ^ ((trow >> 1) & 3);
       kch[j] = 8 * c;
       if (row < BM) {
         const int m = bm0 + row;
@@ -599,12 +599,12 @@ __global__ __launch_bounds__(256) void gemm_nt_dma_kernel(GemmArgs g) {
 #pragma unroll
     for (int i = 0; i < WM; ++i) {
       const int row = wm0 + 16 * i + r;
-      MM::load(fa[i], reinterpret_cast<const bf16_t*>(base + row * ROWB + ((grp ^ ((row >> 2) & 3)) << 4)));
+      MM::load(fa[i], reinterpret_cast<const bf16_t*>(base + row * ROWB + ((grp ^ ((row >> 1) & 3)) << 4)));
     }
 #pragma unroll
     for (int j = 0; j < WN; ++j) {
       const int row = wn0 + 16 * j + r;
-      MM::load(fb[j], reinterpret_cast<const bf16_t*>(base + (BM + row) * ROWB + ((grp ^ ((row >> 2) & 3)) << 4)));
+      MM::load(fb[j], reinterpret_cast<const bf16_t*>(base + (BM + row) * ROWB + ((grp ^ ((row >> 1) & 3)) << 4)));
     }
     MM::mma_tiles(acc, fa, fb);
   }
