@@ -257,6 +257,18 @@ int rr_rkb_fock_cholesky(rr_handle h, int set, size_t n, const double* h_rkb, co
 int rr_rkb_cholesky_to_mo(rr_handle h, int set, const double* c_mo, size_t n_mo, size_t n_negative, double threshold,
                           double* out, size_t capacity, size_t* n_out);
 
+/* ---- Batches of independent molecules (no reference counterpart: many small systems per launch) --
+ * A batch keeps n_mol RI tensors of different sizes resident; rr_batch_jk_factor builds J and K of
+ * D_m = L_m L_m^T for every molecule (L_m n_m x r_m) in one HIP-graph launch (captured on the first
+ * call for a given set of ranks and precision, replayed afterwards). replayed (optional) reports
+ * whether the call replayed a captured graph. Per-molecule results equal rr_ri_jk_factor's. */
+typedef struct rr_batch* rr_batch_t;
+int rr_batch_create(rr_handle h, size_t n_mol, const size_t* n_aux, const size_t* n, const double* const* eri3,
+                    unsigned copies, rr_batch_t* out);
+int rr_batch_destroy(rr_handle h, rr_batch_t batch);
+int rr_batch_jk_factor(rr_handle h, rr_batch_t batch, int precision, const size_t* r, const double* const* lfac,
+                       double* const* j_out, double* const* k_out, int* replayed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,7 @@ EXPORTED = [
     "rr_ri_three_center_tensor", "rr_transform_ri_three_center_ao_legs", "rr_canonical_orthogonalize",
     "rr_ri_mo_fock_rkb", "rr_ri_mo_fock_nonrel",
     "rr_rkb_cholesky_upload", "rr_rkb_cholesky_release", "rr_rkb_fock_cholesky", "rr_rkb_cholesky_to_mo",
+    "rr_batch_create", "rr_batch_destroy", "rr_batch_jk_factor",
 ]
 
 
@@ -566,3 +567,36 @@ class Context:
         self._call("rr_rkb_cholesky_to_mo", ctypes.c_int(set_id), cc.ptr, _sz(nmo), _sz(n_negative),
                    ctypes.c_double(threshold), ctypes.c_void_p(out.ctypes.data), _sz(n_chol), ctypes.byref(nout))
         return out[: nout.value]
+
+    # ---- batches of molecules (one HIP-graph launch per batch build) -------------------------------
+    def batch_create(self, tensors, copies=COPY_FP64):
+        """Keep a list of RI tensors (n_aux, n, n) resident as one batch; returns an opaque handle."""
+        arrs = [_f64(t) for t in tensors]
+        nm = len(arrs)
+        naux = (_sz * nm)(*[t.shape[0] for t in tensors])
+        ns = (_sz * nm)(*[int(round(np.sqrt(np.prod(t.shape[1:])))) for t in tensors])
+        ptrs = (ctypes.c_void_p * nm)(*[a.ptr.value for a in arrs])
+        out = ctypes.c_void_p()
+        self._call("rr_batch_create", _sz(nm), naux, ns, ptrs, ctypes.c_uint(copies), ctypes.byref(out))
+        return (out, [int(x) for x in ns])
+
+    def batch_destroy(self, batch):
+        self._call("rr_batch_destroy", batch[0])
+
+    def batch_jk_factor(self, batch, lfacs, precision=FP64, outs=None):
+        """J, K of D_m = L_m L_m^T for every molecule in one graph launch. Returns ([J_m], [K_m], replayed)."""
+        handle, ns = batch
+        nm = len(ns)
+        ls = [_f64(l) for l in lfacs]
+        r = (_sz * nm)(*[l.shape[1] for l in lfacs])
+        lp = (ctypes.c_void_p * nm)(*[a.ptr.value for a in ls])
+        if outs is None:
+            js = [np.empty((n, n)) for n in ns]
+            ks = [np.empty((n, n)) for n in ns]
+        else:
+            js, ks = outs
+        jp = (ctypes.c_void_p * nm)(*[_Arr(x, np.float64).ptr.value for x in js])
+        kp = (ctypes.c_void_p * nm)(*[_Arr(x, np.float64).ptr.value for x in ks])
+        rep = ctypes.c_int(0)
+        self._call("rr_batch_jk_factor", handle, ctypes.c_int(precision), r, lp, jp, kp, ctypes.byref(rep))
+        return js, ks, bool(rep.value)

@@ -1,0 +1,185 @@
+// librerdmft_gpu, batches of independent molecules: many small RI tensors of different sizes kept
+// resident together, and one J/K build per molecule (factor form, D = L L^T) for the whole batch
+// in a single HIP-graph launch.
+//
+// The first call for a given set of factor ranks and precision runs the batch eagerly (this also
+// sizes the shared workspaces and lets the tile autotuner see every shape), then captures the same
+// sequence of launches into a graph; later calls copy the new factors into the batch's staging
+// buffer, replay the graph and copy the results out. A graph is rebuilt whenever a workspace buffer
+// it recorded has been reallocated since (the context's workspace generation changed).
+#include "rr_core.hpp"
+
+struct rr_batch {
+  std::vector<rr::Tensor> t;
+  std::vector<size_t> l_off, o_off;  // offsets (doubles) into the staging buffers
+  std::vector<size_t> r;             // factor ranks of the captured graph
+  int precision = -1;
+  double* lin = nullptr;             // concatenated factors (device)
+  double* jout = nullptr;            // concatenated J, K (device)
+  double* kout = nullptr;
+  size_t lin_cap = 0, out_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  unsigned long long generation = 0;
+};
+
+namespace {
+
+using rr::Context;
+
+void drop_graph(rr_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void enqueue_all(Context& c, rr_batch* b) {
+  for (size_t m = 0; m < b->t.size(); ++m)
+    rr::enqueue_jk_factor(c, b->t[m], b->precision, b->lin + b->l_off[m], b->r[m], b->jout + b->o_off[m],
+                          b->kout + b->o_off[m]);
+}
+
+}  // namespace
+
+using rr::Error;
+using rr::guarded;
+
+extern "C" {
+
+int rr_batch_create(rr_handle h, size_t n_mol, const size_t* n_aux, const size_t* n, const double* const* eri3,
+                    unsigned copies, rr_batch_t* out) {
+  if (!out) return RR_ERR_ARG;
+  *out = nullptr;
+  rr_batch* b = new rr_batch();
+  const int rc = guarded(h, [&](Context& c) {
+    RR_REQUIRE(n_mol == 0 || (n_aux && n && eri3), RR_ERR_ARG, "batch: null arrays");
+    b->t.resize(n_mol);
+    size_t off = 0;
+    b->o_off.resize(n_mol);
+    for (size_t m = 0; m < n_mol; ++m) {
+      rr::tensor_store(c, b->t[m], eri3[m], n_aux[m], n[m], copies);
+      b->o_off[m] = off;
+      off += n[m] * n[m];
+    }
+    b->out_cap = std::max<size_t>(off, 1);
+    RR_HIP(hipMalloc(&b->jout, b->out_cap * sizeof(double)));
+    RR_HIP(hipMalloc(&b->kout, b->out_cap * sizeof(double)));
+  });
+  if (rc != RR_OK) {
+    for (auto& t : b->t) rr::tensor_free(t);
+    if (b->jout) (void)hipFree(b->jout);
+    if (b->kout) (void)hipFree(b->kout);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return RR_OK;
+}
+
+int rr_batch_destroy(rr_handle h, rr_batch_t b) {
+  if (!h || !b) return RR_ERR_ARG;
+  return guarded(h, [&](Context& c) {
+    RR_HIP(hipStreamSynchronize(c.stream));
+    drop_graph(b);
+    for (auto& t : b->t) rr::tensor_free(t);
+    if (b->lin) (void)hipFree(b->lin);
+    if (b->jout) (void)hipFree(b->jout);
+    if (b->kout) (void)hipFree(b->kout);
+    delete b;
+  });
+}
+
+int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r, const double* const* lfac,
+                       double* const* j_out, double* const* k_out, int* replayed) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(b, RR_ERR_ARG, "batch: null batch");
+    rr::check_prec(precision);
+    const size_t nm = b->t.size();
+    RR_REQUIRE(nm == 0 || (r && lfac && j_out && k_out), RR_ERR_ARG, "batch: null arrays");
+    for (size_t m = 0; m < nm; ++m) {
+      const rr::Tensor& t = b->t[m];
+      RR_REQUIRE(t.f64 || t.f32 || t.b16, RR_ERR_STATE, "batch: empty tensor");
+      (void)rr::tensor_data(t, precision);  // the requested copy exists
+      RR_REQUIRE(r[m] == 0 || lfac[m], RR_ERR_ARG, "batch: null factor");
+    }
+    // staging layout for these ranks
+    std::vector<size_t> loff(nm);
+    size_t ltot = 0;
+    for (size_t m = 0; m < nm; ++m) {
+      loff[m] = ltot;
+      ltot += b->t[m].n * r[m];
+    }
+    const bool same_shape = b->exec && b->precision == precision && b->r == std::vector<size_t>(r, r + nm) &&
+                            b->generation == c.ws_generation;
+    if (!same_shape) {
+      drop_graph(b);
+      if (ltot > b->lin_cap) {
+        RR_HIP(hipStreamSynchronize(c.stream));
+        if (b->lin) RR_HIP(hipFree(b->lin));
+        b->lin = nullptr;
+        RR_HIP(hipMalloc(&b->lin, std::max<size_t>(ltot, 1) * sizeof(double)));
+        b->lin_cap = ltot;
+      }
+      b->l_off = loff;
+      b->r.assign(r, r + nm);
+      b->precision = precision;
+    }
+    // factors into the staging buffer (host arrays are gathered into one transfer)
+    std::vector<double> host;
+    for (size_t m = 0; m < nm; ++m) {
+      const size_t cnt = b->t[m].n * r[m];
+      if (!cnt) continue;
+      if (rr::is_device_ptr(lfac[m])) {
+        RR_HIP(hipMemcpyAsync(b->lin + loff[m], lfac[m], cnt * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      } else {
+        if (host.empty()) host.assign(ltot, 0.0);
+        std::copy(lfac[m], lfac[m] + cnt, host.begin() + loff[m]);
+      }
+    }
+    if (!host.empty()) {
+      // device-pointer factors were copied in place; copy the host part around them
+      for (size_t m = 0; m < nm; ++m) {
+        const size_t cnt = b->t[m].n * r[m];
+        if (cnt && !rr::is_device_ptr(lfac[m]))
+          RR_HIP(hipMemcpyAsync(b->lin + loff[m], host.data() + loff[m], cnt * sizeof(double), hipMemcpyHostToDevice,
+                                c.stream));
+      }
+      RR_HIP(hipStreamSynchronize(c.stream));  // `host` is released at the end of this scope
+    }
+    int rep = 0;
+    if (same_shape) {
+      RR_HIP(hipGraphLaunch(b->exec, c.stream));
+      rep = 1;
+    } else {
+      enqueue_all(c, b);  // eager run: results of this call, workspace sizing, autotuning
+      RR_HIP(hipStreamSynchronize(c.stream));
+      const unsigned long long gen = c.ws_generation;
+      const bool timing = rr::set_phase_timing(false);  // event timing synchronizes: not capturable
+      RR_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+      try {
+        enqueue_all(c, b);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(c.stream, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        rr::set_phase_timing(timing);
+        throw;
+      }
+      rr::set_phase_timing(timing);
+      RR_HIP(hipStreamEndCapture(c.stream, &b->graph));
+      RR_REQUIRE(c.ws_generation == gen, RR_ERR_INTERNAL, "batch: workspace reallocated during graph capture");
+      RR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->generation = gen;
+    }
+    for (size_t m = 0; m < nm; ++m) {
+      const size_t cnt = b->t[m].n * b->t[m].n;
+      if (!cnt) continue;
+      RR_HIP(hipMemcpyAsync(j_out[m], b->jout + b->o_off[m], cnt * sizeof(double), hipMemcpyDefault, c.stream));
+      RR_HIP(hipMemcpyAsync(k_out[m], b->kout + b->o_off[m], cnt * sizeof(double), hipMemcpyDefault, c.stream));
+    }
+    if (replayed) *replayed = rep;
+  });
+}
+
+}  // extern "C"

@@ -61,6 +61,7 @@ struct Context {
   float last_ms[8] = {0};
   RkbChol rkb[kRkbSets];
   void* zpage = nullptr;  // 256 zero bytes (source of out-of-range chunks in direct-to-LDS GEMMs)
+  unsigned long long ws_generation = 0;  // bumped whenever a workspace buffer is (re)allocated
 };
 
 struct Error {
@@ -146,6 +147,13 @@ void mo_tensor_rkb(Context& c, int slot_ll, int slot_ss, size_t nl, size_t ns, c
                    double* dout);
 void mo_tensor_nonrel_embedded(Context& c, int slot, size_t n, const double* dC_spatial, double* dout);
 size_t slot_naux(Context& c, int slot);
+// Tensor storage outside the numbered slots (batches of molecules).
+void tensor_free(Tensor& t);
+void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t n, unsigned copies);
+// J and K of D = L L^T for one tensor, enqueued on the context stream without host synchronization.
+void enqueue_jk_factor(Context& c, const Tensor& t, int prec, const double* dL, size_t r, double* dJ, double* dK);
+// Switches the per-phase HIP-event timing on or off; returns the previous state.
+bool set_phase_timing(bool on);
 
 // Output helper: device pointer if the caller's buffer is device memory, else a workspace buffer
 // that is copied back by finish_out().

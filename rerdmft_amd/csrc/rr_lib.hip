@@ -35,6 +35,7 @@ void* wsbuf(Context& c, int id, size_t bytes) {
   void* p = nullptr;
   RR_HIP(hipMalloc(&p, std::max<size_t>(bytes, 256)));
   c.ws[id] = {p, std::max<size_t>(bytes, 256)};
+  ++c.ws_generation;  // captured graphs that used the old buffers must be rebuilt
   return p;
 }
 
@@ -1746,6 +1747,76 @@ void mo_tensor_nonrel_embedded(Context& c, int slot, size_t n, const double* dC_
 
 size_t slot_naux(Context& c, int slot) { return get_slot(c, slot).naux; }
 
+bool set_phase_timing(bool on) {
+  const bool was = PhaseTimer::enabled;
+  PhaseTimer::enabled = on;
+  return was;
+}
+
+// Allocates t (freeing what it held) and stores eri3 (n_aux, n*n), host or device, in the slot layout.
+void tensor_free(Tensor& t) {
+  if (t.f64) (void)hipFree(t.f64);
+  if (t.f32) (void)hipFree(t.f32);
+  if (t.b16) (void)hipFree(t.b16);
+  t = Tensor();
+}
+
+void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t n, unsigned copies) {
+  RR_REQUIRE(copies & 7u, RR_ERR_ARG, "no tensor copy requested");
+  RR_REQUIRE(eri3 || naux * n == 0, RR_ERR_ARG, "null tensor");
+  RR_HIP(hipStreamSynchronize(c.stream));
+  tensor_free(t);
+  t.naux = naux, t.n = n, t.ld = roundup(std::max<size_t>(n, 1), 8);
+  const size_t cnt = std::max<size_t>(1, naux * n * t.ld);
+  if (copies & RR_COPY_FP64) RR_HIP(hipMalloc(&t.f64, cnt * sizeof(double)));
+  if (copies & RR_COPY_FP32) RR_HIP(hipMalloc(&t.f32, cnt * sizeof(float)));
+  if (copies & RR_COPY_BF16) RR_HIP(hipMalloc(&t.b16, cnt * sizeof(bf16_t)));
+  if (naux == 0 || n == 0) return;
+  const bool dev = is_device_ptr(eri3);
+  const size_t rowbytes = n * n * sizeof(double);
+  size_t chunk = std::max<size_t>(1, std::min<size_t>(naux, (size_t(256) << 20) / rowbytes));
+  chunk = std::min<size_t>(chunk, 65535);
+  double* stage = dev ? nullptr : static_cast<double*>(wsbuf(c, WS_STAGE, chunk * rowbytes));
+  for (size_t p0 = 0; p0 < naux; p0 += chunk) {
+    const size_t rows = std::min(chunk, naux - p0);
+    const double* src = eri3 + p0 * n * n;
+    if (!dev) {
+      RR_HIP(hipMemcpyAsync(stage, src, rows * rowbytes, hipMemcpyHostToDevice, c.stream));
+      src = stage;
+    }
+    dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32), (unsigned)rows);
+    if (t.f64) hipLaunchKernelGGL(k_store_rows<double>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f64 + p0 * n * t.ld);
+    if (t.f32) hipLaunchKernelGGL(k_store_rows<float>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f32 + p0 * n * t.ld);
+    if (t.b16) hipLaunchKernelGGL(k_store_rows<bf16_t>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.b16 + p0 * n * t.ld);
+    RR_HIP(hipGetLastError());
+    if (!dev) RR_HIP(hipStreamSynchronize(c.stream));
+  }
+}
+
+// J and K of D = L L^T (L device, n x r) on the context stream with no host synchronization: the
+// Coulomb term is taken from the exchange half-transform (gamma from W). Outputs device n x n.
+void enqueue_jk_factor(Context& c, const Tensor& t, int prec, const double* dL, size_t r, double* dJ, double* dK) {
+  const size_t n = t.n;
+  if (r == 0 || t.naux == 0) {
+    hipLaunchKernelGGL(k_zero, dim3(grid_for(n * n)), dim3(256), 0, c.stream, dJ, n * n);
+    hipLaunchKernelGGL(k_zero, dim3(grid_for(n * n)), dim3(256), 0, c.stream, dK, n * n);
+    RR_HIP(hipGetLastError());
+    return;
+  }
+  double* lt = static_cast<double*>(wsbuf(c, WS_T1, 2 * r * n * sizeof(double)));
+  hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(r * n)), dim3(256), 0, c.stream, dL, r, 1, r, r, n, n, 1.0, lt);
+  const size_t rp = roundup(r, 8);
+  double* dfac = static_cast<double*>(wsbuf(c, WS_DENS2, n * rp * sizeof(double)));
+  hipLaunchKernelGGL(k_build_dfac, dim3(grid_for(n * rp)), dim3(256), 0, c.stream, lt, (const double*)nullptr, r, n, n, rp,
+                     1, 1.0, 0.0, dfac);
+  RR_HIP(hipGetLastError());
+  double* gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
+  ExchangeOut ex = exchange_real_prec(c, t, prec, lt, nullptr, r, n, dfac, gamma);
+  double* jpart = nullptr;
+  const int jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart);
+  real_assemble(c, t, jpart, jsplit, &ex, -0.5, nullptr, nullptr, dJ, dK);
+}
+
 }  // namespace rr
 
 // ================================================================================================
@@ -1863,29 +1934,7 @@ static void alloc_tensor(Context& c, rr::Tensor& t, size_t naux, size_t n, unsig
 int rr_tensor_upload(rr_handle h, int slot, const double* eri3, size_t n_aux, size_t n, unsigned copies) {
   return guarded(h, [&](Context& c) {
     RR_REQUIRE(slot >= 0 && slot < rr::kSlots, RR_ERR_ARG, "tensor slot out of range");
-    RR_REQUIRE(eri3 || n_aux * n == 0, RR_ERR_ARG, "null tensor");
-    rr::Tensor& t = c.slot[slot];
-    alloc_tensor(c, t, n_aux, n, copies);
-    if (n_aux == 0 || n == 0) return;
-    const bool dev = rr::is_device_ptr(eri3);
-    const size_t rowbytes = n * n * sizeof(double);
-    size_t chunk = std::max<size_t>(1, std::min<size_t>(n_aux, (size_t(256) << 20) / rowbytes));
-    chunk = std::min<size_t>(chunk, 65535);
-    double* stage = dev ? nullptr : static_cast<double*>(rr::wsbuf(c, rr::WS_STAGE, chunk * rowbytes));
-    for (size_t p0 = 0; p0 < n_aux; p0 += chunk) {
-      const size_t rows = std::min(chunk, n_aux - p0);
-      const double* src = eri3 + p0 * n * n;
-      if (!dev) {
-        RR_HIP(hipMemcpyAsync(stage, src, rows * rowbytes, hipMemcpyHostToDevice, c.stream));
-        src = stage;
-      }
-      dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32), (unsigned)rows);
-      if (t.f64) hipLaunchKernelGGL(rr::k_store_rows<double>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f64 + p0 * n * t.ld);
-      if (t.f32) hipLaunchKernelGGL(rr::k_store_rows<float>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f32 + p0 * n * t.ld);
-      if (t.b16) hipLaunchKernelGGL(rr::k_store_rows<rr::bf16_t>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.b16 + p0 * n * t.ld);
-      RR_HIP(hipGetLastError());
-      if (!dev) RR_HIP(hipStreamSynchronize(c.stream));
-    }
+    rr::tensor_store(c, c.slot[slot], eri3, n_aux, n, copies);
   });
 }
 
