@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--delta-precision", default="bf16", choices=["bf16", "f32", "f64"])
     p.add_argument("--cpu-sample", type=int, default=1500, help="aux rows timed on the CPU (0: skip)")
     p.add_argument("--no-fp64-leg", action="store_true")
+    p.add_argument("--batch-molecules", type=int, default=256,
+                   help="molecules in the uneven-batch leg (0: skip); n ~ U[20,200], n_aux = 4n, n_occ = n/5")
     p.add_argument("--copy-probe-gib", type=float, default=0.0,
                    help="only run a device copy of this many GiB (calibrates HBM byte counters) and exit")
     return p.parse_args()
@@ -279,10 +281,48 @@ def main():
         "phase_ms_mixed": {f"{k[1]}[{k[0]}]": round(v[0], 3) for k, v in phases_p.items() if v[1]},
         "cpu_baseline": cpu,
         "setup_s": setup_s,
+        "batch_uneven_molecules": batch_leg(torch, rr, ctx, a.batch_molecules) if a.batch_molecules > 0 else None,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def batch_leg(torch, rr, ctx, nmol, reps=5):
+    """Uneven batch: nmol molecules, n ~ U[20,200], n_aux = 4n, n_occ = n/5; J/K of D = L L^T for every
+    molecule in one HIP-graph launch (FP64). The first call runs the batch eagerly and captures it."""
+    g = np.random.default_rng(20261203 + 5)
+    ns = [int(x) for x in g.integers(20, 201, nmol)]
+    dev = torch.device("cuda:0")
+    tens = []
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20261208)
+    for n in ns:
+        t = torch.randn((4 * n, n, n), device=dev, dtype=torch.float64, generator=gen)
+        tens.append(0.5 * (t + t.transpose(1, 2)))
+    batch = ctx.batch_create(tens, rr.COPY_FP64)
+    del tens
+    ls = [torch.randn((n, max(1, n // 5)), device=dev, dtype=torch.float64, generator=gen) for n in ns]
+    js = [torch.empty((n, n), device=dev, dtype=torch.float64) for n in ns]
+    ks = [torch.empty((n, n), device=dev, dtype=torch.float64) for n in ns]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.batch_jk_factor(batch, ls, outs=(js, ks))  # eager run + capture
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    ctx.batch_jk_factor(batch, ls, outs=(js, ks))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, _, rep = ctx.batch_jk_factor(batch, ls, outs=(js, ks))
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / reps
+    flops = sum(2.0 * (4 * n) * n * n * max(1, n // 5) + (4 * n) * n * (n + 1) * max(1, n // 5) for n in ns)
+    ctx.batch_destroy(batch)
+    return {"molecules": nmol, "n_range": [min(ns), max(ns)], "dtype": "f64",
+            "builds_per_s_graph": nmol / t_graph, "ms_per_batch_graph": 1e3 * t_graph,
+            "ms_first_call_eager_plus_capture": 1e3 * t_first, "replayed": bool(rep),
+            "exchange_TFLOP_per_s": flops / t_graph / 1e12}
 
 
 def cpu_baseline(n, nocc, nsample, naux, c0):

@@ -5,12 +5,24 @@
 // The first call for a given set of factor ranks and precision runs the batch eagerly (this also
 // sizes the shared workspaces and lets the tile autotuner see every shape), then captures the same
 // sequence of launches into a graph; later calls copy the new factors into the batch's staging
-// buffer, replay the graph and copy the results out. A graph is rebuilt whenever a workspace buffer
-// it recorded has been reallocated since (the context's workspace generation changed).
+// buffer, replay the graph and copy the results out. Molecules run on up to eight parallel graph
+// branches, each with a private stream and workspace set, so small molecules overlap. A graph is
+// rebuilt whenever a workspace buffer it recorded has been reallocated since (a branch's workspace
+// generation changed).
+#include <algorithm>
+
 #include "rr_core.hpp"
+
+// One branch of the batch graph: a private stream and workspace set (kernel launches of different
+// branches share nothing but the read-only tensors and disjoint output regions).
+struct BatchLane {
+  rr::Context ctx;
+  hipEvent_t done = nullptr;
+};
 
 struct rr_batch {
   std::vector<rr::Tensor> t;
+  std::vector<BatchLane*> lanes;
   std::vector<size_t> l_off, o_off;  // offsets (doubles) into the staging buffers
   std::vector<size_t> r;             // factor ranks of the captured graph
   int precision = -1;
@@ -20,12 +32,14 @@ struct rr_batch {
   size_t lin_cap = 0, out_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  unsigned long long generation = 0;
+  std::vector<unsigned long long> generation;  // lane workspace generations at capture
+  hipEvent_t fork = nullptr;
 };
 
 namespace {
 
 using rr::Context;
+using rr::Error;
 
 void drop_graph(rr_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -34,15 +48,58 @@ void drop_graph(rr_batch* b) {
   b->graph = nullptr;
 }
 
+constexpr size_t kMaxLanes = 8;
+
+void free_lane(BatchLane* l) {
+  if (!l) return;
+  (void)hipStreamSynchronize(l->ctx.stream);
+  for (auto& kv : l->ctx.ws) (void)hipFree(kv.second.first);
+  if (l->ctx.zpage) (void)hipFree(l->ctx.zpage);
+  if (l->done) (void)hipEventDestroy(l->done);
+  if (l->ctx.stream) (void)hipStreamDestroy(l->ctx.stream);
+  delete l;
+}
+
+BatchLane* make_lane(int device) {
+  BatchLane* l = new BatchLane();
+  l->ctx.device = device;
+  if (hipStreamCreateWithFlags(&l->ctx.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&l->done, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc(&l->ctx.zpage, 256) != hipSuccess || hipMemset(l->ctx.zpage, 0, 256) != hipSuccess) {
+    free_lane(l);
+    throw Error{RR_ERR_HIP, "batch: cannot create a graph branch"};
+  }
+  return l;
+}
+
+// Molecules are dealt round-robin to the lanes, largest first, each lane in order on its stream.
+// The lanes fork from and join back into the main stream through events (captured as graph edges).
 void enqueue_all(Context& c, rr_batch* b) {
-  for (size_t m = 0; m < b->t.size(); ++m)
-    rr::enqueue_jk_factor(c, b->t[m], b->precision, b->lin + b->l_off[m], b->r[m], b->jout + b->o_off[m],
+  RR_HIP(hipEventRecord(b->fork, c.stream));
+  std::vector<size_t> order(b->t.size());
+  for (size_t m = 0; m < order.size(); ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return b->t[x].n > b->t[y].n; });
+  for (BatchLane* l : b->lanes) RR_HIP(hipStreamWaitEvent(l->ctx.stream, b->fork, 0));
+  for (size_t i = 0; i < order.size(); ++i) {
+    const size_t m = order[i];
+    Context& lc = b->lanes[i % b->lanes.size()]->ctx;
+    rr::enqueue_jk_factor(lc, b->t[m], b->precision, b->lin + b->l_off[m], b->r[m], b->jout + b->o_off[m],
                           b->kout + b->o_off[m]);
+  }
+  for (BatchLane* l : b->lanes) {
+    RR_HIP(hipEventRecord(l->done, l->ctx.stream));
+    RR_HIP(hipStreamWaitEvent(c.stream, l->done, 0));
+  }
+}
+
+std::vector<unsigned long long> lane_generations(const rr_batch* b) {
+  std::vector<unsigned long long> g;
+  for (const BatchLane* l : b->lanes) g.push_back(l->ctx.ws_generation);
+  return g;
 }
 
 }  // namespace
 
-using rr::Error;
 using rr::guarded;
 
 extern "C" {
@@ -65,8 +122,13 @@ int rr_batch_create(rr_handle h, size_t n_mol, const size_t* n_aux, const size_t
     b->out_cap = std::max<size_t>(off, 1);
     RR_HIP(hipMalloc(&b->jout, b->out_cap * sizeof(double)));
     RR_HIP(hipMalloc(&b->kout, b->out_cap * sizeof(double)));
+    RR_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    const size_t nl = std::max<size_t>(1, std::min(kMaxLanes, n_mol));
+    for (size_t i = 0; i < nl; ++i) b->lanes.push_back(make_lane(c.device));
   });
   if (rc != RR_OK) {
+    for (BatchLane* l : b->lanes) free_lane(l);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (auto& t : b->t) rr::tensor_free(t);
     if (b->jout) (void)hipFree(b->jout);
     if (b->kout) (void)hipFree(b->kout);
@@ -82,6 +144,8 @@ int rr_batch_destroy(rr_handle h, rr_batch_t b) {
   return guarded(h, [&](Context& c) {
     RR_HIP(hipStreamSynchronize(c.stream));
     drop_graph(b);
+    for (BatchLane* l : b->lanes) free_lane(l);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (auto& t : b->t) rr::tensor_free(t);
     if (b->lin) (void)hipFree(b->lin);
     if (b->jout) (void)hipFree(b->jout);
@@ -111,7 +175,7 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
       ltot += b->t[m].n * r[m];
     }
     const bool same_shape = b->exec && b->precision == precision && b->r == std::vector<size_t>(r, r + nm) &&
-                            b->generation == c.ws_generation;
+                            b->generation == lane_generations(b);
     if (!same_shape) {
       drop_graph(b);
       if (ltot > b->lin_cap) {
@@ -154,7 +218,7 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
     } else {
       enqueue_all(c, b);  // eager run: results of this call, workspace sizing, autotuning
       RR_HIP(hipStreamSynchronize(c.stream));
-      const unsigned long long gen = c.ws_generation;
+      const std::vector<unsigned long long> gen = lane_generations(b);
       const bool timing = rr::set_phase_timing(false);  // event timing synchronizes: not capturable
       RR_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
       try {
@@ -168,7 +232,7 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
       }
       rr::set_phase_timing(timing);
       RR_HIP(hipStreamEndCapture(c.stream, &b->graph));
-      RR_REQUIRE(c.ws_generation == gen, RR_ERR_INTERNAL, "batch: workspace reallocated during graph capture");
+      RR_REQUIRE(lane_generations(b) == gen, RR_ERR_INTERNAL, "batch: workspace reallocated during graph capture");
       RR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->generation = gen;
     }
