@@ -269,6 +269,25 @@ int rr_batch_destroy(rr_handle h, rr_batch_t batch);
 int rr_batch_jk_factor(rr_handle h, rr_batch_t batch, int precision, const size_t* r, const double* const* lfac,
                        double* const* j_out, double* const* k_out, int* replayed);
 
+/* ---- SCF starting-density network (no reference counterpart; its fallback is the core-Hamiltonian
+ * guess of main.cpp:5248-5250) -------------------------------------------------------------------
+ * A fully connected FP32 network n_in -> n_layers x (n_hidden, tanh) -> n_out (linear), trained on
+ * the device. Parameters are laid out per layer as W [fan_out][fan_in] row-major, then b [fan_out].
+ * Inputs and targets are row-major double arrays [batch][n_in] / [batch][n_out] (host or device).
+ * The loss is 1/(2 batch n_out) sum (y - target)^2; rr_mlp_train_step applies one Adam step
+ * (beta1 0.9, beta2 0.999, eps 1e-8) and returns the loss before it. */
+typedef struct rr_mlp* rr_mlp_t;
+int rr_mlp_create(rr_handle h, int n_in, int n_hidden, int n_layers, int n_out, unsigned long long seed, rr_mlp_t* out);
+int rr_mlp_destroy(rr_handle h, rr_mlp_t mlp);
+int rr_mlp_num_params(rr_handle h, rr_mlp_t mlp, size_t* n);
+int rr_mlp_get_params(rr_handle h, rr_mlp_t mlp, float* out);
+int rr_mlp_set_params(rr_handle h, rr_mlp_t mlp, const float* in);
+int rr_mlp_forward(rr_handle h, rr_mlp_t mlp, size_t batch, const double* x, double* y);
+int rr_mlp_loss_grad(rr_handle h, rr_mlp_t mlp, size_t batch, const double* x, const double* target, double* loss,
+                     float* grads);
+int rr_mlp_train_step(rr_handle h, rr_mlp_t mlp, size_t batch, const double* x, const double* target, double lr,
+                      double* loss);
+
 #ifdef __cplusplus
 }
 #endif

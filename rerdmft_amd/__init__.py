@@ -37,6 +37,8 @@ EXPORTED = [
     "rr_ri_mo_fock_rkb", "rr_ri_mo_fock_nonrel",
     "rr_rkb_cholesky_upload", "rr_rkb_cholesky_release", "rr_rkb_fock_cholesky", "rr_rkb_cholesky_to_mo",
     "rr_batch_create", "rr_batch_destroy", "rr_batch_jk_factor",
+    "rr_mlp_create", "rr_mlp_destroy", "rr_mlp_num_params", "rr_mlp_get_params", "rr_mlp_set_params",
+    "rr_mlp_forward", "rr_mlp_loss_grad", "rr_mlp_train_step",
 ]
 
 
@@ -600,3 +602,54 @@ class Context:
         rep = ctypes.c_int(0)
         self._call("rr_batch_jk_factor", handle, ctypes.c_int(precision), r, lp, jp, kp, ctypes.byref(rep))
         return js, ks, bool(rep.value)
+
+    # ---- SCF starting-density network ----------------------------------------------------------------
+    def mlp_create(self, n_in, n_hidden, n_layers, n_out, seed=0):
+        """FP32 network n_in -> n_layers x (n_hidden, tanh) -> n_out on the device; returns a handle."""
+        out = ctypes.c_void_p()
+        self._call("rr_mlp_create", ctypes.c_int(n_in), ctypes.c_int(n_hidden), ctypes.c_int(n_layers),
+                   ctypes.c_int(n_out), ctypes.c_ulonglong(seed), ctypes.byref(out))
+        return out
+
+    def mlp_destroy(self, mlp):
+        self._call("rr_mlp_destroy", mlp)
+
+    def mlp_num_params(self, mlp):
+        n = _sz()
+        self._call("rr_mlp_num_params", mlp, ctypes.byref(n))
+        return n.value
+
+    def mlp_get_params(self, mlp):
+        p = np.empty(self.mlp_num_params(mlp), dtype=np.float32)
+        self._call("rr_mlp_get_params", mlp, ctypes.c_void_p(p.ctypes.data))
+        return p
+
+    def mlp_set_params(self, mlp, params):
+        a = _Arr(params, np.float32)
+        if a.obj.size != self.mlp_num_params(mlp):
+            raise ValueError("parameter vector has the wrong length")
+        self._call("rr_mlp_set_params", mlp, a.ptr)
+
+    def mlp_forward(self, mlp, x, n_out):
+        xa = _f64(x)
+        b = x.shape[0]
+        y = np.empty((b, n_out))
+        self._call("rr_mlp_forward", mlp, _sz(b), xa.ptr, ctypes.c_void_p(y.ctypes.data))
+        return y
+
+    def mlp_loss_grad(self, mlp, x, target):
+        """Loss 1/(2 B n_out) sum (y - t)^2 and its gradient with respect to the parameter vector."""
+        xa, ta = _f64(x), _f64(target)
+        g = np.empty(self.mlp_num_params(mlp), dtype=np.float32)
+        loss = ctypes.c_double()
+        self._call("rr_mlp_loss_grad", mlp, _sz(x.shape[0]), xa.ptr, ta.ptr, ctypes.byref(loss),
+                   ctypes.c_void_p(g.ctypes.data))
+        return loss.value, g
+
+    def mlp_train_step(self, mlp, x, target, lr):
+        """One Adam step on (x, target); returns the loss before the step."""
+        xa, ta = _f64(x), _f64(target)
+        loss = ctypes.c_double()
+        self._call("rr_mlp_train_step", mlp, _sz(x.shape[0]), xa.ptr, ta.ptr, ctypes.c_double(lr),
+                   ctypes.byref(loss))
+        return loss.value

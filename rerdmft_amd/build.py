@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librerdmft_gpu.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rr_lib.hip", "rr_sigma.hip", "rr_transform.hip", "rr_rkb_chol.hip", "rr_batch.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rr_lib.hip", "rr_sigma.hip", "rr_transform.hip", "rr_rkb_chol.hip", "rr_batch.hip", "rr_mlp.hip")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("rr_device.hpp", "rr_gemm.hpp", "rr_core.hpp")] + [
     os.path.join(ROOT, "include", "rerdmft_gpu.h")]
 
