@@ -9,7 +9,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librerdmft_gpu.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rr_lib.hip", "rr_sigma.hip", "rr_transform.hip", "rr_rkb_chol.hip", "rr_batch.hip", "rr_mlp.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rr_lib.hip", "rr_sigma.hip", "rr_transform.hip", "rr_rkb_chol.hip",
+                                                       "rr_batch.hip", "rr_mlp.hip", "rr_gemm_wide.hip")]
+# translation units whose matrix-core accumulators stay in the AGPR form (wide tiles)
+AGPR_FORM = {"rr_gemm_wide.hip"}
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("rr_device.hpp", "rr_gemm.hpp", "rr_core.hpp")] + [
     os.path.join(ROOT, "include", "rerdmft_gpu.h")]
 
@@ -36,7 +39,11 @@ def build_library(verbose=False):
     procs, objs = [], []
     for src in SOURCES:
         obj = os.path.join(obj_dir, os.path.splitext(os.path.basename(src))[0] + ".o")
-        cmd = [HIPCC] + compile_flags + ["-c", src, "-o", obj]
+        flags = compile_flags
+        if os.path.basename(src) in AGPR_FORM:
+            i = compile_flags.index("-amdgpu-mfma-vgpr-form=1")
+            flags = compile_flags[:i - 1] + compile_flags[i + 1:]  # drop the "-mllvm" pair
+        cmd = [HIPCC] + flags + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd), cmd))

@@ -82,6 +82,13 @@ struct Error {
   } while (0)
 
 inline size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
+// Row stride of a stored 3-index tensor: a multiple of 32 elements (so that every 64-byte bf16
+// chunk a GEMM stage loads lies inside one 128-byte cache line) when that pads by at most 1/16,
+// else a multiple of 8 (16-byte vectors).
+inline size_t tensor_ld(size_t n) {
+  const size_t m = n > 0 ? n : 1, a = roundup(m, 32);
+  return (a - m) * 16 <= m ? a : roundup(m, 8);
+}
 
 
 // Named grow-only device workspace buffers.

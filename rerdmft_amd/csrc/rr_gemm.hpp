@@ -634,4 +634,166 @@ __global__ __launch_bounds__(256) void gemm_nt_dma_kernel(GemmArgs g) {
     }
 }
 
+
+// Leaner variant of gemm_nt_dma_kernel with the per-stage bookkeeping taken off the hot path:
+//  * rows beyond M (N) are clamped to row M-1 (N-1): their products land in accumulators the
+//    epilogue never stores, so only the k-range needs the zero page, and only in a partial stage;
+//  * a full stage issues each DMA from its precomputed row pointer with one 64-bit add;
+//  * the wait before a stage is a compile-time count per wave class (waves holding TI/4 + 1 or TI/4
+//    DMA slots), and vmcnt(0) in the last NST-2 stages.
+// The staged data, LDS layout and MFMA sequence are those of gemm_nt_dma_kernel (bitwise equal).
+template <int N>
+__device__ inline void wait_vm() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+template <typename TO, int BM, int BN, int WGM, int NST>
+__global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
+  using MM = Mma<bf16_t>;
+  using acc_t = typename MM::acc_t;
+  using frag = typename MM::frag;
+  constexpr int BK = 32;
+  constexpr int ROWB = 64;
+  constexpr int TI = (BM + BN) / 16;
+  constexpr int STAGE = (BM + BN) * ROWB;
+  constexpr int MAXJ = (TI + 3) / 4;
+  constexpr int WGN = 4 / WGM;
+  constexpr int WM = BM / (16 * WGM), WN = BN / (16 * WGN);
+  constexpr int CHI = MAXJ, CLO = TI / 4;  // DMA slots of waves 0..TI%4-1 and of the others
+  static_assert(BM % 16 == 0 && BN % 16 == 0, "tiles are made of 16-row DMA groups");
+  static_assert(WM * 16 * WGM == BM && WN * 16 * WGN == BN, "tile not divisible over the waves");
+  static_assert(NST >= 2 && CHI * (NST - 2) <= 63, "wait counts must fit vmcnt");
+
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[NST * STAGE];
+
+  const int total = g.ntiles * g.nbatch * g.nsplit;
+  int w = static_cast<int>(blockIdx.x);
+  if (g.xcd_remap) w = remap_xcd(w, total);
+  const int tile = w % g.ntiles;
+  const int rest = w / g.ntiles;
+  const int batch = rest % g.nbatch;
+  const int split = rest / g.nbatch;
+  int tm, tn;
+  if (g.tri) {
+    int t = tile;
+    int a = static_cast<int>((sqrtf(8.0f * static_cast<float>(t) + 1.0f) - 1.0f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    while (a * (a + 1) / 2 > t) --a;
+    tm = a;
+    tn = t - a * (a + 1) / 2;
+  } else {
+    tm = tile % g.tiles_m;
+    tn = tile / g.tiles_m;
+  }
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = lane & 15, grp = lane >> 4;
+  const int wm0 = (wave % WGM) * (16 * WM), wn0 = (wave / WGM) * (16 * WN);
+  const int bm0 = tm * BM, bn0 = tn * BN;
+
+  const long long kbeg = static_cast<long long>(split) * g.kper;
+  long long kend = kbeg + g.kper;
+  if (kend > g.K) kend = g.K;
+
+  const bf16_t* Ab = static_cast<const bf16_t*>(g.A) + static_cast<long long>(batch) * g.sAb;
+  const bf16_t* Bb = static_cast<const bf16_t*>(g.B) + static_cast<long long>(batch) * g.sBb;
+
+  const bool hi = wave < (TI & 3) || (TI & 3) == 0;  // this wave holds CHI slots (else CLO)
+  const bf16_t* src[MAXJ];
+  int kch[MAXJ];
+#pragma unroll
+  for (int j = 0; j < MAXJ; ++j) {
+    const int i = wave + 4 * j;
+    const int row = 16 * min(i, TI - 1) + (lane >> 2);
+    const int trow = row < BM ? row : row - BM;
+    const int c = (lane & 3) ^ ((trow >> 1) & 3);
+    kch[j] = 8 * c;
+    if (row < BM) {
+      const int m = min(bm0 + row, g.M - 1);
+      src[j] = Ab + static_cast<long long>(m) * g.lda + 8 * c + kbeg;
+    } else {
+      const int n = min(bn0 + trow, g.N - 1);
+      src[j] = Bb + static_cast<long long>(n) * g.ldb + 8 * c + kbeg;
+    }
+  }
+  const unsigned lbase = __builtin_amdgcn_readfirstlane(lds_addr(lds));
+  const int nstage = static_cast<int>((kend - kbeg + BK - 1) / BK);
+  const int nfull = static_cast<int>((kend - kbeg) / BK);
+  auto issue = [&](int st) {
+    const unsigned sb = lbase + static_cast<unsigned>((st % NST) * STAGE);
+    const long long koff = static_cast<long long>(st) * BK;
+    if (st < nfull) {
+#pragma unroll
+      for (int j = 0; j < MAXJ; ++j)
+        if (j < CLO || (j < CHI && hi)) lds_dma16(src[j] + koff, sb + static_cast<unsigned>((wave + 4 * j) * 1024));
+    } else {
+#pragma unroll
+      for (int j = 0; j < MAXJ; ++j)
+        if (j < CLO || (j < CHI && hi)) {
+          const void* gp = (kbeg + koff + kch[j] < kend) ? static_cast<const void*>(src[j] + koff) : g.zpage;
+          lds_dma16(gp, sb + static_cast<unsigned>((wave + 4 * j) * 1024));
+        }
+    }
+  };
+
+  acc_t acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = acc_t{};
+
+  for (int st = 0; st < NST - 1 && st < nstage; ++st) issue(st);
+  for (int st = 0; st < nstage; ++st) {
+    if (st + NST - 2 < nstage) {  // NST-2 later stages of this wave are in flight
+      if (hi)
+        wait_vm<CHI * (NST - 2)>();
+      else
+        wait_vm<CLO * (NST - 2)>();
+    } else {
+      wait_vm<0>();
+    }
+    __builtin_amdgcn_s_barrier();
+    if (st + NST - 1 < nstage) issue(st + NST - 1);
+    const unsigned char* base = lds + (st % NST) * STAGE;
+    frag fa[WM], fb[WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      const int row = wm0 + 16 * i + r;
+      MM::load(fa[i], reinterpret_cast<const bf16_t*>(base + row * ROWB + ((grp ^ ((row >> 1) & 3)) << 4)));
+    }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int row = wn0 + 16 * j + r;
+      MM::load(fb[j], reinterpret_cast<const bf16_t*>(base + (BM + row) * ROWB + ((grp ^ ((row >> 1) & 3)) << 4)));
+    }
+    MM::mma_tiles(acc, fa, fb);
+  }
+
+  const long long coff = static_cast<long long>(batch) * g.sCb + static_cast<long long>(split) * g.sCz;
+  TO* Cb = static_cast<TO*>(g.C) + coff;
+  TO* Cb2 = g.C2 ? static_cast<TO*>(g.C2) + coff : nullptr;
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int n = bn0 + wn0 + 16 * j + r;
+      if (n >= g.N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = bm0 + wm0 + 16 * i + MM::out_row(grp, e);
+        if (m >= g.M) continue;
+        int mm = m, nn = n;
+        TO* base = Cb;
+        if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
+        if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        TO* p = base + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
+        double v = g.alpha * static_cast<double>(acc[i][j][e]);
+        if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
+        store_out<TO>(p, v);
+      }
+    }
+}
+
 }  // namespace rr

@@ -2,7 +2,7 @@
 // C ABI (include/rerdmft_gpu.h).
 //
 // Device layout of a 3-index tensor slot (an RI tensor (n_aux, n*n) or a set of Cholesky vectors):
-//   Bt[P][nu][mu] = B[P][mu][nu]   (each aux row transposed), leading dimension ld = roundup(n, 8),
+//   Bt[P][nu][mu] = B[P][mu][nu]   (each aux row transposed), leading dimension ld = tensor_ld(n),
 //   columns [n, ld) zero-filled, kept in any of fp64 / fp32 / bf16 (all copies resident in HBM).
 // With this layout every contraction of the path is the "NT" form C = A B^T of rr_gemm.hpp:
 //   exchange half-transform  W[nu][(P,i)] = sum_mu Bt_P[nu][mu] Lt[i][mu]      (A = Bt_P, B = Lt)
@@ -277,6 +277,74 @@ __global__ __launch_bounds__(256) void k_coulomb(const TB* B, size_t rowlen, siz
     }
 }
 
+// Upper triangle of a symmetric tensor row in 16-byte vectors: row nu holds the vectors from the one
+// containing the diagonal (index floor(nu / VN)) to the end of the padded row. Rows come in groups of
+// VN with W - g vectors each (group g, W = ld / VN), so the vectors before group g number
+// S(g) = VN (g W - g (g - 1) / 2); tri_offset maps a flat vector index to its element offset.
+__device__ inline size_t tri_offset(size_t v, size_t W, size_t VN, size_t ld) {
+  const double b = 2.0 * static_cast<double>(W) + 1.0;
+  const double disc = b * b - 8.0 * static_cast<double>(v) / static_cast<double>(VN);
+  size_t g = static_cast<size_t>((b - sqrt(disc > 0.0 ? disc : 0.0)) * 0.5);
+  auto S = [&](size_t x) { return VN * (x * W) - VN * ((x * (x - 1)) / 2); };
+  while (g > 0 && S(g) > v) --g;
+  while (g + 1 <= W && S(g + 1) <= v) ++g;
+  const size_t off = v - S(g), len = W - g;
+  return (g * VN + off / len) * ld + (g + off % len) * VN;
+}
+// Vectors of the upper triangle of an n x ld row.
+inline size_t tri_vectors(size_t n, size_t ld, size_t VN) {
+  const size_t W = ld / VN;
+  size_t t = 0;
+  for (size_t nu = 0; nu < n; ++nu) t += W - nu / VN;
+  return t;
+}
+
+// k_coulomb over the upper triangle only (symmetric tensors): jpart[split][e] for the triangle
+// elements e; the rest of jpart is left unwritten.
+template <typename TB, int U>
+__global__ __launch_bounds__(256) void k_coulomb_tri(const TB* B, size_t rowlen, size_t ld, size_t nvec, size_t naux,
+                                                     const double* gamma, size_t p_per_split, double* jpart) {
+  using V = Vec16<TB>;
+  constexpr int VN = V::N;
+  const size_t W = ld / VN;
+  const size_t vbase = (size_t)blockIdx.x * (256 * U) + threadIdx.x;
+  const size_t split = blockIdx.y;
+  const size_t pb = split * p_per_split, pe = std::min(naux, pb + p_per_split);
+  size_t e[U];
+  bool valid[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const size_t v = vbase + (size_t)u * 256;
+    valid[u] = v < nvec;
+    e[u] = valid[u] ? tri_offset(v, W, VN, ld) : 0;
+  }
+  double acc[U][VN];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int q = 0; q < VN; ++q) acc[u][q] = 0.0;
+  for (size_t P = pb; P < pe; ++P) {
+    const double g = gamma[P];
+    const TB* row = B + P * rowlen;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (valid[u]) {
+        double bv[VN];
+        V::load(row + e[u], bv);
+#pragma unroll
+        for (int q = 0; q < VN; ++q) acc[u][q] += g * bv[q];
+      }
+    }
+  }
+  double* out = jpart + split * rowlen;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (valid[u]) {
+#pragma unroll
+      for (int q = 0; q < VN; ++q) out[e[u] + q] = acc[u][q];
+    }
+}
+
 // ================================================================================================
 // Epilogues
 // ================================================================================================
@@ -292,6 +360,7 @@ struct RealAsm {
   size_t kstride = 0;
   int ktile = 0;   // tile size of the tri layout (0: full G)
   int ksym = 0;    // full G only: 1 -> K = G + G^T, 0 -> K = G
+  int jtri = 0;    // jpart holds the upper triangle only (row <= column)
   double kfac = -0.5;
   const double* h = nullptr;
   double* F = nullptr;
@@ -304,8 +373,10 @@ __global__ void k_real_asm(RealAsm a) {
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const size_t mu = idx / a.n, nu = idx % a.n;
     double j = 0.0;
-    if (a.jpart)
-      for (int s = 0; s < a.jsplit; ++s) j += a.jpart[s * a.jstride + nu * a.ld + mu];
+    if (a.jpart) {
+      const size_t src = a.jtri ? (mu < nu ? mu * a.ld + nu : nu * a.ld + mu) : nu * a.ld + mu;
+      for (int s = 0; s < a.jsplit; ++s) j += a.jpart[s * a.jstride + src];
+    }
     double k = 0.0;
     if (a.kpart) {
       if (a.ktile > 0) {
@@ -421,8 +492,16 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               {128, 208, 4, 1}, {256, 208, 4, 1}, {128, 128, 2, 1}, {64, 208, 4, 1},
                               // multi-stage direct-to-LDS (bf16): 12-16
                               {128, 208, 4, 1}, {128, 208, 4, 1}, {64, 208, 4, 1}, {128, 128, 2, 1},
-                              {256, 128, 2, 1}};
+                              {256, 128, 2, 1},
+                              // direct-to-LDS built with AGPR accumulators (rr_gemm_wide.hip): 17-19
+                              {128, 416, 2, 1}, {256, 224, 2, 1}, {128, 208, 4, 1},
+                              // lean direct-to-LDS (gemm_nt_dma2_kernel): 20-22 here, 23-24 AGPR form
+                              {128, 208, 4, 1}, {256, 128, 2, 1}, {128, 128, 2, 1}, {128, 208, 4, 1},
+                              {128, 416, 2, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
+
+// rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
+bool launch_dma_agpr(int v, bool out_f64, const GemmArgs& g, int grid, hipStream_t s);
 
 template <typename TI, typename TO>
 void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
@@ -456,7 +535,12 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 14: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 64, 208, 4, 4>), dim3(grid), dim3(256), 0, s, g); return;
           case 15: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 128, 128, 2, 4>), dim3(grid), dim3(256), 0, s, g); return;
           case 16: hipLaunchKernelGGL((gemm_nt_dma_kernel<TO, 256, 128, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
-          default: break;
+          case 20: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 208, 4, 3>), dim3(grid), dim3(256), 0, s, g); return;
+          case 21: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 128, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
+          case 22: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 128, 2, 4>), dim3(grid), dim3(256), 0, s, g); return;
+          default:
+            if (launch_dma_agpr(v, std::is_same<TO, double>::value, g, grid, s)) return;
+            break;
         }
       }
       throw Error{RR_ERR_ARG, "LDS tile configuration not available for this precision"};
@@ -773,14 +857,16 @@ void gamma_stream(Context& c, const Tensor& t, const TB* dDt, double* gamma, boo
 
 // jpart[s][e] partial sums of Jt = sum_P gamma_P Bt_P; returns the number of splits.
 template <typename TB>
-int coulomb_stream(Context& c, const Tensor& t, const double* gamma, double* jpart_out_or_null, double** jpart_used) {
+int coulomb_stream(Context& c, const Tensor& t, const double* gamma, double* jpart_out_or_null, double** jpart_used,
+                   bool tri = false) {
   const TB* B = tensor_ptr<TB>(t);
   RR_REQUIRE(B, RR_ERR_STATE, "tensor slot has no copy in the requested precision");
   const size_t rowlen = t.n * t.ld;
   constexpr int U = 4;
   const size_t vn = 16 / sizeof(TB);
+  const size_t nvec = tri ? tri_vectors(t.n, t.ld, vn) : 0;
   const size_t per_wg = 256 * vn * U;
-  const size_t nx = (rowlen + per_wg - 1) / per_wg;
+  const size_t nx = tri ? (nvec + 256 * U - 1) / (256 * U) : (rowlen + per_wg - 1) / per_wg;
   size_t nsplit = std::max<size_t>(1, std::min<size_t>(32, (2048 + nx - 1) / nx));
   nsplit = std::min(nsplit, std::max<size_t>(1, t.naux));
   const size_t pps = (t.naux + nsplit - 1) / nsplit;
@@ -789,8 +875,12 @@ int coulomb_stream(Context& c, const Tensor& t, const double* gamma, double* jpa
                                     : static_cast<double*>(wsbuf(c, WS_JPART, std::max<size_t>(1, nsplit) * rowlen * sizeof(double)));
   {
     PhaseTimer pt(c, PH_COUL);
-    hipLaunchKernelGGL((k_coulomb<TB, U>), dim3((unsigned)nx, (unsigned)nsplit), dim3(256), 0, c.stream, B, rowlen, t.naux,
-                       gamma, pps, jpart);
+    if (tri)
+      hipLaunchKernelGGL((k_coulomb_tri<TB, U>), dim3((unsigned)std::max<size_t>(nx, 1), (unsigned)nsplit), dim3(256), 0,
+                         c.stream, B, rowlen, t.ld, nvec, t.naux, gamma, pps, jpart);
+    else
+      hipLaunchKernelGGL((k_coulomb<TB, U>), dim3((unsigned)nx, (unsigned)nsplit), dim3(256), 0, c.stream, B, rowlen,
+                         t.naux, gamma, pps, jpart);
     RR_HIP(hipGetLastError());
   }
   *jpart_used = jpart;
@@ -916,7 +1006,7 @@ ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* d
     g.lda = ld, g.sAb = n * ld;
     g.B = lt, g.ldb = ld, g.sBb = 0;
     g.C = W, g.ldcm = kdim, g.ldcn = 1, g.sCb = rp;
-    g.M = (int)n, g.N = (int)(nfac * rp), g.K = (int)ld, g.nbatch = (int)naux;
+    g.M = (int)n, g.N = (int)(nfac * rp), g.K = (int)roundup(n, 8), g.nbatch = (int)naux;  // columns >= n are zero
     if (dRt) g.C2 = W2, g.ncut = (int)rp;
     GemmSpec sp;
     sp.prec = prec, sp.out = prec, sp.cfg = gemm1_cfg(c, nfac * rp);
@@ -961,20 +1051,21 @@ ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* d
 
 // J partial sums for a real density D (device fp64, n x n): returns jpart and split count.
 template <typename TB>
-int coulomb_real(Context& c, const Tensor& t, const double* dD, double** jpart) {
+int coulomb_real(Context& c, const Tensor& t, const double* dD, double** jpart, bool tri) {
   TB* dt = static_cast<TB*>(wsbuf(c, WS_DT, t.n * t.ld * sizeof(TB)));
   hipLaunchKernelGGL((k_pack<TB>), dim3(grid_for(t.n * t.ld)), dim3(256), 0, c.stream, dD, t.n, 1, t.n, t.n, t.n, t.ld, 1.0,
                      dt);
   RR_HIP(hipGetLastError());
   double* gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
   gamma_stream<TB>(c, t, dt, gamma, false);
-  return coulomb_stream<TB>(c, t, gamma, nullptr, jpart);
+  return coulomb_stream<TB>(c, t, gamma, nullptr, jpart, tri);
 }
 
-int coulomb_real_prec(Context& c, const Tensor& t, int prec, const double* dD, double** jpart) {
-  if (prec == P64) return coulomb_real<double>(c, t, dD, jpart);
-  if (prec == P32) return coulomb_real<float>(c, t, dD, jpart);
-  return coulomb_real<bf16_t>(c, t, dD, jpart);
+// tri: write only the upper triangle of Jt (symmetric tensors; read back with RealAsm::jtri).
+int coulomb_real_prec(Context& c, const Tensor& t, int prec, const double* dD, double** jpart, bool tri = false) {
+  if (prec == P64) return coulomb_real<double>(c, t, dD, jpart, tri);
+  if (prec == P32) return coulomb_real<float>(c, t, dD, jpart, tri);
+  return coulomb_real<bf16_t>(c, t, dD, jpart, tri);
 }
 
 ExchangeOut exchange_real_prec(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
@@ -985,17 +1076,17 @@ ExchangeOut exchange_real_prec(Context& c, const Tensor& t, int prec, const doub
 }
 
 // Coulomb stream only (gamma given): returns split partials of Jt.
-int coulomb_from_gamma(Context& c, const Tensor& t, int prec, const double* gamma, double** jpart) {
-  if (prec == P64) return coulomb_stream<double>(c, t, gamma, nullptr, jpart);
-  if (prec == P32) return coulomb_stream<float>(c, t, gamma, nullptr, jpart);
-  return coulomb_stream<bf16_t>(c, t, gamma, nullptr, jpart);
+int coulomb_from_gamma(Context& c, const Tensor& t, int prec, const double* gamma, double** jpart, bool tri = false) {
+  if (prec == P64) return coulomb_stream<double>(c, t, gamma, nullptr, jpart, tri);
+  if (prec == P32) return coulomb_stream<float>(c, t, gamma, nullptr, jpart, tri);
+  return coulomb_stream<bf16_t>(c, t, gamma, nullptr, jpart, tri);
 }
 
 void real_assemble(Context& c, const Tensor& t, double* jpart, int jsplit, const ExchangeOut* ex, double kfac,
-                   const double* dh, double* dF, double* dJ, double* dK) {
+                   const double* dh, double* dF, double* dJ, double* dK, bool jtri) {
   PhaseTimer pt(c, PH_ASM);
   RealAsm a;
-  a.n = t.n, a.ld = t.ld;
+  a.n = t.n, a.ld = t.ld, a.jtri = jtri ? 1 : 0;
   // Reduce the split partials first with coalesced reads, so that the assembly's transposed reads
   // touch one matrix instead of every partial.
   if (jpart && jsplit > 1) {
@@ -1170,8 +1261,8 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   ex.ksym = 1;
   double* jpart = nullptr;
   int jsplit = 0;
-  if (dJ) jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart);
-  real_assemble(c, t, jpart, jsplit, dK ? &ex : nullptr, 0.0, nullptr, nullptr, dJ, dK);
+  if (dJ) jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart, t.symmetric);
+  real_assemble(c, t, jpart, jsplit, dK ? &ex : nullptr, 0.0, nullptr, nullptr, dJ, dK, t.symmetric);
 }
 
 
@@ -1231,7 +1322,7 @@ ExchangeOut exchange_flavors(Context& c, int prec, const std::vector<Flavor>& fl
     g.B = lt, g.ldb = t.ld;
     g.C = W + f0.off * ldw, g.C2 = W + f1.off * ldw, g.ncut = (int)(2 * rp);
     g.ldcm = ldw, g.ldcn = 1, g.sCb = 2 * rp;
-    g.M = (int)t.n, g.N = (int)(4 * rp), g.K = (int)t.ld, g.nbatch = (int)naux;
+    g.M = (int)t.n, g.N = (int)(4 * rp), g.K = (int)roundup(t.n, 8), g.nbatch = (int)naux;
     GemmSpec sp;
     sp.prec = prec, sp.out = prec, sp.cfg = gemm1_cfg(c, 4 * rp);
     sp.lds = gemm1_lds(c, t.n, 4 * rp);
@@ -1387,7 +1478,7 @@ void mo_real_slot(Context& c, const Tensor& t, int prec, const double* dC, size_
       g.A = ct, g.lda = ld;
       g.B = static_cast<const TI*>(tensor_data(t, prec)) + p0 * n * ld, g.ldb = ld, g.sBb = n * ld;
       g.C = s1, g.ldcm = ld, g.ldcn = 1, g.sCb = per;
-      g.M = (int)nmo, g.N = (int)n, g.K = (int)ld, g.nbatch = (int)rows;
+      g.M = (int)nmo, g.N = (int)n, g.K = (int)roundup(n, 8), g.nbatch = (int)rows;
       GemmSpec sp;
       sp.prec = op, sp.out = op, sp.cfg = 2;
       gemm_auto(c, sp, g);
@@ -1484,7 +1575,7 @@ void mo_cplx_slot(Context& c, const std::vector<Flavor>& fl, int prec, const dou
       g.B = static_cast<const TI*>(tensor_data(t, prec)) + p0 * t.n * t.ld, g.ldb = t.ld, g.sBb = t.n * t.ld;
       g.C = s1 + 2 * f.off, g.C2 = s1 + 2 * f.off + f.fs, g.mcut = (int)nt;
       g.ldcm = ldk, g.ldcn = 1, g.sCb = per;
-      g.M = (int)(2 * nt), g.N = (int)t.n, g.K = (int)t.ld, g.nbatch = (int)rows;
+      g.M = (int)(2 * nt), g.N = (int)t.n, g.K = (int)roundup(t.n, 8), g.nbatch = (int)rows;
       GemmSpec sp;
       sp.prec = op, sp.out = op, sp.cfg = 2;
       gemm_auto(c, sp, g);
@@ -1761,12 +1852,21 @@ void tensor_free(Tensor& t) {
   t = Tensor();
 }
 
+// flag[0] = 1 if some row block of src (rows x n x n) is not exactly symmetric.
+__global__ void k_sym_check(const double* src, size_t rows, size_t n, int* flag) {
+  const size_t nn = n * n, total = rows * nn;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t P = idx / nn, rem = idx % nn, mu = rem / n, nu = rem % n;
+    if (nu > mu && !(src[idx] == src[P * nn + nu * n + mu])) flag[0] = 1;
+  }
+}
+
 void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t n, unsigned copies) {
   RR_REQUIRE(copies & 7u, RR_ERR_ARG, "no tensor copy requested");
   RR_REQUIRE(eri3 || naux * n == 0, RR_ERR_ARG, "null tensor");
   RR_HIP(hipStreamSynchronize(c.stream));
   tensor_free(t);
-  t.naux = naux, t.n = n, t.ld = roundup(std::max<size_t>(n, 1), 8);
+  t.naux = naux, t.n = n, t.ld = tensor_ld(n);
   const size_t cnt = std::max<size_t>(1, naux * n * t.ld);
   if (copies & RR_COPY_FP64) RR_HIP(hipMalloc(&t.f64, cnt * sizeof(double)));
   if (copies & RR_COPY_FP32) RR_HIP(hipMalloc(&t.f32, cnt * sizeof(float)));
@@ -1777,6 +1877,9 @@ void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t
   size_t chunk = std::max<size_t>(1, std::min<size_t>(naux, (size_t(256) << 20) / rowbytes));
   chunk = std::min<size_t>(chunk, 65535);
   double* stage = dev ? nullptr : static_cast<double*>(wsbuf(c, WS_STAGE, chunk * rowbytes));
+  // Exactly symmetric blocks (mu nu|P) = (nu mu|P) let the Coulomb stream read only the upper triangle.
+  int* asym = static_cast<int*>(wsbuf(c, WS_TMP9 + 77, sizeof(int)));
+  RR_HIP(hipMemsetAsync(asym, 0, sizeof(int), c.stream));
   for (size_t p0 = 0; p0 < naux; p0 += chunk) {
     const size_t rows = std::min(chunk, naux - p0);
     const double* src = eri3 + p0 * n * n;
@@ -1784,13 +1887,18 @@ void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t
       RR_HIP(hipMemcpyAsync(stage, src, rows * rowbytes, hipMemcpyHostToDevice, c.stream));
       src = stage;
     }
-    dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32), (unsigned)rows);
+    dim3 grid((unsigned)((t.ld + 31) / 32), (unsigned)((n + 31) / 32), (unsigned)rows);  // every padding column
     if (t.f64) hipLaunchKernelGGL(k_store_rows<double>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f64 + p0 * n * t.ld);
     if (t.f32) hipLaunchKernelGGL(k_store_rows<float>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f32 + p0 * n * t.ld);
     if (t.b16) hipLaunchKernelGGL(k_store_rows<bf16_t>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.b16 + p0 * n * t.ld);
+    hipLaunchKernelGGL(k_sym_check, dim3(grid_for(rows * n * n)), dim3(256), 0, c.stream, src, rows, n, asym);
     RR_HIP(hipGetLastError());
     if (!dev) RR_HIP(hipStreamSynchronize(c.stream));
   }
+  int h_asym = 0;
+  RR_HIP(hipMemcpyAsync(&h_asym, asym, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  t.symmetric = h_asym == 0;
 }
 
 // J and K of D = L L^T (L device, n x r) on the context stream with no host synchronization: the
@@ -1813,8 +1921,8 @@ void enqueue_jk_factor(Context& c, const Tensor& t, int prec, const double* dL, 
   double* gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
   ExchangeOut ex = exchange_real_prec(c, t, prec, lt, nullptr, r, n, dfac, gamma);
   double* jpart = nullptr;
-  const int jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart);
-  real_assemble(c, t, jpart, jsplit, &ex, -0.5, nullptr, nullptr, dJ, dK);
+  const int jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart, t.symmetric);
+  real_assemble(c, t, jpart, jsplit, &ex, -0.5, nullptr, nullptr, dJ, dK, t.symmetric);
 }
 
 }  // namespace rr
@@ -1924,7 +2032,7 @@ static void alloc_tensor(Context& c, rr::Tensor& t, size_t naux, size_t n, unsig
   if (t.f32) RR_HIP(hipFree(t.f32));
   if (t.b16) RR_HIP(hipFree(t.b16));
   t = rr::Tensor();
-  t.naux = naux, t.n = n, t.ld = rr::roundup(std::max<size_t>(n, 1), 8);
+  t.naux = naux, t.n = n, t.ld = rr::tensor_ld(n);
   const size_t cnt = std::max<size_t>(1, naux * n * t.ld);
   if (copies & RR_COPY_FP64) RR_HIP(hipMalloc(&t.f64, cnt * sizeof(double)));
   if (copies & RR_COPY_FP32) RR_HIP(hipMalloc(&t.f32, cnt * sizeof(float)));
@@ -1981,7 +2089,7 @@ int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double
                "rr_ri_jk_factor: J without a density needs the factor and the exchange build");
     if (j_out && density) {
       const double* dD = rr::in_buf(c, rr::WS_P, density, n * n);
-      jsplit = rr::coulomb_real_prec(c, t, precision, dD, &jpart);
+      jsplit = rr::coulomb_real_prec(c, t, precision, dD, &jpart, t.symmetric);
     }
     rr::ExchangeOut ex;
     if (k_out && r > 0) {
@@ -2008,12 +2116,12 @@ int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double
         gamma = static_cast<double*>(rr::wsbuf(c, rr::WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
       }
       ex = rr::exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma);
-      if (fused_j) jsplit = rr::coulomb_from_gamma(c, t, precision, gamma, &jpart);
+      if (fused_j) jsplit = rr::coulomb_from_gamma(c, t, precision, gamma, &jpart, t.symmetric);
     }
     rr::OutBuf oj = rr::out_buf(c, rr::WS_OUT_J, j_out, n * n), ok = rr::out_buf(c, rr::WS_OUT_K, k_out, n * n);
     if (k_out && r == 0) hipLaunchKernelGGL(rr::k_zero, dim3(rr::grid_for(n * n)), dim3(256), 0, c.stream, ok.dev, n * n);
     rr::real_assemble(c, t, jpart, jsplit, (k_out && r > 0) ? &ex : nullptr, -0.5, nullptr, nullptr, oj.dev,
-                      (k_out && r > 0) ? ok.dev : nullptr);
+                      (k_out && r > 0) ? ok.dev : nullptr, t.symmetric);
     rr::finish_out(c, oj);
     rr::finish_out(c, ok);
   });
@@ -2033,7 +2141,7 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
     const double* dS = rr::in_buf(c, rr::WS_S, overlap, nl * nl);
     const double* dX = rr::in_buf(c, rr::WS_X, x_large, nl * nx);
     double* jpart = nullptr;
-    const int jsplit = rr::coulomb_real_prec(c, t, precision, dP, &jpart);
+    const int jsplit = rr::coulomb_real_prec(c, t, precision, dP, &jpart, t.symmetric);
     // Recovery: D = P/2, C_occ = X Y_occ; c_t = sqrt(2) C_occ^T = sqrt(2) Y_occ^T X^T.
     const double* rows = nullptr;
     std::vector<double> occ;
@@ -2047,7 +2155,8 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
     rr::OutBuf of = rr::out_buf(c, rr::WS_OUT_F, fock, nl * nl), oj = rr::out_buf(c, rr::WS_OUT_J, j_out, nl * nl),
                ok = rr::out_buf(c, rr::WS_OUT_K, k_out, nl * nl);
     if (ok.dev && n_occ == 0) hipLaunchKernelGGL(rr::k_zero, dim3(rr::grid_for(nl * nl)), dim3(256), 0, c.stream, ok.dev, nl * nl);
-    rr::real_assemble(c, t, jpart, jsplit, n_occ > 0 ? &ex : nullptr, -0.5, dH, of.dev, oj.dev, n_occ > 0 ? ok.dev : nullptr);
+    rr::real_assemble(c, t, jpart, jsplit, n_occ > 0 ? &ex : nullptr, -0.5, dH, of.dev, oj.dev, n_occ > 0 ? ok.dev : nullptr,
+                      t.symmetric);
     rr::finish_out(c, of);
     rr::finish_out(c, oj);
     rr::finish_out(c, ok);

@@ -177,7 +177,7 @@ def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
     ctx.upload(9, b, rr.COPY_BF16)
     outs = []
     try:
-        for cfg in [1, 2, 5, 100, 101] + list(range(102, 117)):
+        for cfg in [1, 2, 5, 100, 101] + list(range(102, 125)):
             ctx.set_tile_config(cfg)
             dj, dk = ctx.ri_jk_delta(9, c_new, c_old, precision=rr.BF16)
             j, k = ctx.ri_jk_factor(9, None, l, precision=rr.BF16)
@@ -187,3 +187,21 @@ def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
     for cfg, dk, k in outs[1:]:
         assert np.array_equal(dk, outs[0][1]), f"incremental K differs for configuration {cfg}"
         assert np.array_equal(k, outs[0][2]), f"K differs for configuration {cfg}"
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+@pytest.mark.parametrize("n", [37, 130])
+def test_coulomb_upper_triangle_and_full_stream(ctx, symmetric, n):
+    """An exactly symmetric tensor is streamed over its upper triangle only; a non-symmetric one over
+    every element. Both must give J = sum_P tr(B_P D) B_P (D symmetric), in every precision."""
+    naux = 23
+    b = syn.ri_tensor(naux, n, 0.7, 6.0, 900 + n, symmetric=symmetric)
+    d = syn.symmetric(n, 901 + n)
+    gamma = np.einsum("pmn,nm->p", b, d)
+    ref = np.einsum("p,pmn->mn", gamma, b)
+    ctx.upload(6, b, rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16)
+    for prec, tol in [(rr.FP64, 1e-12), (rr.FP32, 1e-5), (rr.BF16, 3e-2)]:
+        j, _ = ctx.ri_jk_factor(6, d, None, precision=prec, want_k=False)
+        assert np.max(np.abs(j - ref)) <= tol * np.max(np.abs(ref)), (prec, symmetric)
+        if prec == rr.FP64 and symmetric:
+            assert np.array_equal(j, j.T)
