@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--no-fp64-leg", action="store_true")
     p.add_argument("--batch-molecules", type=int, default=256,
                    help="molecules in the uneven-batch leg (0: skip); n ~ U[20,200], n_aux = 4n, n_occ = n/5")
+    p.add_argument("--sigma-densities", type=int, default=64,
+                   help="trial densities in the shared-tensor leg (0: skip); n = 800 (100 occ + 700 virt), n_aux = 3200")
     p.add_argument("--copy-probe-gib", type=float, default=0.0,
                    help="only run a device copy of this many GiB (calibrates HBM byte counters) and exit")
     return p.parse_args()
@@ -282,6 +284,7 @@ def main():
         "cpu_baseline": cpu,
         "setup_s": setup_s,
         "batch_uneven_molecules": batch_leg(torch, rr, ctx, a.batch_molecules) if a.batch_molecules > 0 else None,
+        "sigma_trial_densities": sigma_leg(torch, rr, ctx, a.sigma_densities) if a.sigma_densities > 0 else None,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -323,6 +326,48 @@ def batch_leg(torch, rr, ctx, nmol, reps=5):
             "builds_per_s_graph": nmol / t_graph, "ms_per_batch_graph": 1e3 * t_graph,
             "ms_first_call_eager_plus_capture": 1e3 * t_first, "replayed": bool(rep),
             "exchange_TFLOP_per_s": flops / t_graph / 1e12}
+
+
+def sigma_leg(torch, rr, ctx, nd, reps=2, slot=14):
+    """Shared-tensor batch: nd orbital-Hessian trial densities D_k = C_occ T_k C_virt^T (T_k N(0,1),
+    n_occ = 100, n_virt = 700, n = 800, n_aux = 3200) built in one graph-captured call per precision;
+    J_k and K_k of every density. Inputs and outputs stay in HBM."""
+    n, no, naux = 800, 100, 3200
+    dev = torch.device("cuda:0")
+    ctx.generate(slot, naux, n, 1.0, 40.0, SEED + 7, rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20261209)
+    q, _ = torch.linalg.qr(torch.randn((n, n), device=dev, dtype=torch.float64, generator=gen))
+    c_occ, c_virt = q[:, :no].contiguous(), q[:, no:].contiguous()
+    t = torch.randn((nd, no, n - no), device=dev, dtype=torch.float64, generator=gen)
+    rfacs = torch.matmul(c_virt.unsqueeze(0), t.transpose(1, 2)).contiguous()  # R_k = C_virt T_k^T
+    js = torch.empty((nd, n, n), device=dev, dtype=torch.float64)
+    ks = torch.empty((nd, n, n), device=dev, dtype=torch.float64)
+    out = {"densities": nd, "n": n, "n_occ": no, "n_aux": naux}
+    ref = None
+    for pname, prec in (("f64", rr.FP64), ("f32", rr.FP32), ("bf16", rr.BF16)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.ri_jk_multi(slot, c_occ, rfacs, precision=prec, j_out=js, k_out=ks)  # eager run + capture
+        torch.cuda.synchronize()
+        t_first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, _, rep = ctx.ri_jk_multi(slot, c_occ, rfacs, precision=prec, j_out=js, k_out=ks)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res = {"builds_per_s": nd / dt, "ms_per_call": 1e3 * dt, "ms_first_call_eager_plus_capture": 1e3 * t_first,
+               "replayed": bool(rep)}
+        if ref is None:
+            ref = (js.clone(), ks.clone())
+        else:
+            res["max_rel_dJ_vs_f64"] = float((js - ref[0]).abs().max() / ref[0].abs().max())
+            res["max_rel_dK_vs_f64"] = float((ks - ref[1]).abs().max() / ref[1].abs().max())
+        out[pname] = res
+    # exchange flops per density: half-transform of R_k and the rank-k product (B L once per call)
+    out["exchange_TFLOP_per_density"] = 2 * 2.0 * naux * n * n * no / 1e12
+    ctx.release(slot)
+    return out
 
 
 def cpu_baseline(n, nocc, nsample, naux, c0):

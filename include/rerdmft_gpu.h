@@ -80,6 +80,16 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
 int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double* density, const double* lfac,
                     const double* rfac, size_t r, double* j_out, double* k_out);
 
+/* Many densities sharing the tensor and the left factor (no reference counterpart; the trial
+ * densities of the orbital-Hessian sigma products, D_k = L R_k^T with L the occupied orbitals):
+ * for k < n_dens, J_k = sum_P tr(B_P D_k) B_P and K_k = sum_P B_P^T (L R_k^T) B_P, as n_dens calls of
+ * rr_ri_jk_factor(density = NULL, lfac, rfac + k n r) would give. lfac is n x r, rfac n_dens
+ * consecutive n x r blocks, j_out / k_out n_dens consecutive n x n blocks (either may be NULL).
+ * B L is formed once; the first call for a shape runs eagerly and captures a HIP graph of the whole
+ * build, which later calls of the same shape replay (replayed, optional, reports it). */
+int rr_ri_jk_multi(rr_handle h, int slot, int precision, size_t n, size_t r, const double* lfac, size_t n_dens,
+                   const double* rfac, double* j_out, double* k_out, int* replayed);
+
 /* Incremental build for D = 2 C C^T (no reference counterpart; replaces a full call of
  * NonRelFockMatrixRi.cpp:27-100 between two SCF iterations): dJ = J(D_new - D_old),
  * dK = K(D_new - D_old) from the aligned-difference factorization of D_new - D_old, so the

@@ -25,7 +25,7 @@ PRECISION_NAMES = {FP64: "f64", FP32: "f32", BF16: "bf16"}
 EXPORTED = [
     "rr_create", "rr_destroy", "rr_last_error", "rr_set_tile_config", "rr_timing",
     "rr_tensor_upload", "rr_tensor_generate", "rr_tensor_info", "rr_tensor_release",
-    "rr_nonrel_fock_ri", "rr_ri_jk_factor", "rr_ri_jk_delta", "rr_x2c_fock_ri",
+    "rr_nonrel_fock_ri", "rr_ri_jk_factor", "rr_ri_jk_multi", "rr_ri_jk_delta", "rr_x2c_fock_ri",
     "rr_ri_fock_two_electron_direct", "rr_rkb_fock_ri", "rr_build_ri_mo_eri", "rr_ri_mo_eri_rotated",
     "rr_ri_mo_eri_positive_block", "rr_build_ri_nonrel_spin_mo_eri", "rr_ri_nonrel_spin_mo_eri_rotated",
     "rr_ri_nonrel_spin_embedded_b", "rr_nonrel_fock_cholesky", "rr_x2c_fock_cholesky",
@@ -212,6 +212,19 @@ class Context:
         self._call("rr_ri_jk_factor", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), d.ptr, l.ptr, rr_.ptr,
                    _sz(r), jp, kp)
         return j, k
+
+    def ri_jk_multi(self, slot, lfac, rfacs, precision=FP64, want_j=True, want_k=True, j_out=None, k_out=None):
+        """J_k, K_k of D_k = L R_k^T for a stack of right factors rfacs (n_dens, n, r), sharing L (n, r).
+        Returns (J (n_dens, n, n) or None, K or None, replayed)."""
+        n, r = lfac.shape
+        nd = rfacs.shape[0]
+        l, rf = _f64(lfac), _f64(rfacs)
+        j, jp = _out(j_out, (nd, n, n), np.float64) if want_j else (None, ctypes.c_void_p(0))
+        k, kp = _out(k_out, (nd, n, n), np.float64) if want_k else (None, ctypes.c_void_p(0))
+        rep = ctypes.c_int(0)
+        self._call("rr_ri_jk_multi", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), _sz(r), l.ptr, _sz(nd), rf.ptr,
+                   jp, kp, ctypes.byref(rep))
+        return j, k, bool(rep.value)
 
     def ri_jk_delta(self, slot, c_new, c_old, precision=FP64, dj_out=None, dk_out=None):
         """dJ, dK of D_new - D_old with D = 2 C C^T (aligned-difference factorization)."""

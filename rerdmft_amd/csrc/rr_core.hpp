@@ -46,6 +46,15 @@ struct RkbChol {
   double* tr = nullptr;
 };
 
+// A captured HIP graph together with the shape it was recorded for and the workspace generation
+// its buffers belong to.
+struct GraphPlan {
+  std::vector<long long> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  unsigned long long gen = 0;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -62,6 +71,7 @@ struct Context {
   RkbChol rkb[kRkbSets];
   void* zpage = nullptr;  // 256 zero bytes (source of out-of-range chunks in direct-to-LDS GEMMs)
   unsigned long long ws_generation = 0;  // bumped whenever a workspace buffer is (re)allocated
+  GraphPlan multi_plan;                  // rr_ri_jk_multi
 };
 
 struct Error {

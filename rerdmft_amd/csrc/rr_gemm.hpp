@@ -36,6 +36,10 @@ struct GemmArgs {
   // written to C2 at (m - mcut, n) or (m, n - ncut) with the same strides.
   void* C2 = nullptr;
   int mcut = 0, ncut = 0;
+  // Column blocks: with nblk > 0, column n goes to block n / nblk (at sCnb logical elements from
+  // block 0) as column n % nblk -- several outputs of one launch that share A.
+  int nblk = 0;
+  long long sCnb = 0;
   // k-batched reduction: the contraction runs over kbatch blocks of K each; block b of A (B) starts
   // sAk (sBk) logical elements after block 0. kper then counts flat k over kbatch * K.
   int kbatch = 0;
@@ -248,6 +252,11 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
         TO* base = Cb;
         if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
         if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        if (g.nblk > 0) {
+          const int b = nn / g.nblk;
+          nn -= b * g.nblk;
+          base += static_cast<long long>(b) * g.sCnb * CW;
+        }
         TO* p = base + (static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn) * CW;
         double v = g.alpha * static_cast<double>(acc[i][j][e]);
         if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
@@ -433,6 +442,11 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
         TO* base = Cb;
         if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
         if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        if (g.nblk > 0) {
+          const int b = nn / g.nblk;
+          nn -= b * g.nblk;
+          base += static_cast<long long>(b) * g.sCnb;
+        }
         TO* p = base + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
         double v = g.alpha * static_cast<double>(acc[i][j][e]);
         if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
@@ -626,6 +640,11 @@ __global__ __launch_bounds__(256) void gemm_nt_dma_kernel(GemmArgs g) {
         TO* base = Cb;
         if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
         if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        if (g.nblk > 0) {
+          const int b = nn / g.nblk;
+          nn -= b * g.nblk;
+          base += static_cast<long long>(b) * g.sCnb;
+        }
         TO* p = base + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
         double v = g.alpha * static_cast<double>(acc[i][j][e]);
         if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
@@ -788,6 +807,11 @@ __global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
         TO* base = Cb;
         if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, base = Cb2;
         if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, base = Cb2;
+        if (g.nblk > 0) {
+          const int b = nn / g.nblk;
+          nn -= b * g.nblk;
+          base += static_cast<long long>(b) * g.sCnb;
+        }
         TO* p = base + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
         double v = g.alpha * static_cast<double>(acc[i][j][e]);
         if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);

@@ -622,7 +622,7 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
     run_gemm(c, sp, g);
     return;
   }
-  std::vector<long long> key = {sp.prec, sp.out, sp.cplx, g.M, g.N, g.K, g.nbatch, g.nsplit, g.tri, g.kbatch, g.ncut, g.mcut};
+  std::vector<long long> key = {sp.prec, sp.out, sp.cplx, g.M, g.N, g.K, g.nbatch, g.nsplit, g.tri, g.kbatch, g.ncut, g.mcut, g.nblk};
   auto it = c.tuned.find(key);
   if (it != c.tuned.end()) {
     sp.cfg = it->second.first, sp.lds = it->second.second;
@@ -1263,6 +1263,186 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   int jsplit = 0;
   if (dJ) jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart, t.symmetric);
   real_assemble(c, t, jpart, jsplit, dK ? &ex : nullptr, 0.0, nullptr, nullptr, dJ, dK, t.symmetric);
+}
+
+
+// ================================================================================================
+// Many densities sharing the tensor and the left factor, D_k = L R_k^T (the trial densities of the
+// orbital-Hessian sigma products: L the occupied orbitals, R_k = C_virt T_k^T). W_L = B L is built
+// once; the R_k are half-transformed G at a time in one launch (column blocks of the output); then
+// per density gamma_k = <W_L, R_k> and K_k = sum_P W_L,P W_Rk,P^T, and J_k = sum_P gamma_k[P] B_P
+// for all densities in passes of 8 over the tensor.
+// ================================================================================================
+enum { WS_MU_LT = 200000, WS_MU_WL, WS_MU_RT, WS_MU_WR, WS_MU_DF, WS_MU_GP, WS_MU_GAM, WS_MU_KP, WS_MU_JP,
+       WS_MU_IN_L, WS_MU_IN_R, WS_MU_OUT_J, WS_MU_OUT_K };
+
+// jpart[split][k][e] = sum over the split's aux rows of gamma[k][P] Bt[P][e] for nk <= NK densities,
+// over the upper-triangle vectors (tri) or every vector of the row.
+template <typename TB, int NK>
+__global__ __launch_bounds__(256) void k_coulomb_multi(const TB* B, size_t rowlen, size_t ld, size_t nvec, int tri,
+                                                       size_t naux, const double* gamma, size_t gstride, int nk,
+                                                       size_t pps, double* jpart) {
+  using V = Vec16<TB>;
+  constexpr int VN = V::N;
+  const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nvec) return;  // no barriers below
+  const size_t split = blockIdx.y;
+  const size_t pb = split * pps, pe = std::min(naux, pb + pps);
+  const size_t e = tri ? tri_offset(v, ld / VN, VN, ld) : v * VN;
+  double acc[NK][VN];
+#pragma unroll
+  for (int k = 0; k < NK; ++k)
+#pragma unroll
+    for (int q = 0; q < VN; ++q) acc[k][q] = 0.0;
+  for (size_t P = pb; P < pe; ++P) {
+    double bv[VN];
+    V::load(B + P * rowlen + e, bv);
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      if (k < nk) {
+        const double g = gamma[k * gstride + P];
+#pragma unroll
+        for (int q = 0; q < VN; ++q) acc[k][q] += g * bv[q];
+      }
+    }
+  }
+  double* out = jpart + split * (size_t)NK * rowlen;
+#pragma unroll
+  for (int k = 0; k < NK; ++k)
+    if (k < nk) {
+#pragma unroll
+      for (int q = 0; q < VN; ++q) out[k * rowlen + e + q] = acc[k][q];
+    }
+}
+
+// J_k[mu][nu] = sum_s jpart[s][k][.] at (min, max) (tri) or (nu, mu) of the transposed layout.
+__global__ void k_multi_jasm(const double* jpart, int nsplit, size_t sstride, size_t rowlen, size_t ld, size_t n, int nk,
+                             int tri, double* J) {
+  const size_t nn = n * n, total = (size_t)nk * nn;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t k = idx / nn, rem = idx % nn, mu = rem / n, nu = rem % n;
+    const size_t src = tri ? (mu < nu ? mu * ld + nu : nu * ld + mu) : nu * ld + mu;
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += jpart[z * sstride + k * rowlen + src];
+    J[idx] = s;
+  }
+}
+
+template <typename TI>
+void multi_jk(Context& c, const Tensor& t, int prec, const double* dL, const double* dR, size_t r, size_t nd,
+              double* dJ, double* dK) {
+  const size_t n = t.n, ld = t.ld, naux = t.naux;
+  const size_t rp = roundup(std::max<size_t>(r, 1), 8), kdim = naux * rp;
+  const size_t G = std::min(nd, std::max<size_t>(1, 416 / rp));  // densities per half-transform launch
+  const int tri = t.symmetric ? 1 : 0;
+  // W_L[nu][(P,i)] = (B_P^T L)[nu][i]
+  TI* lt = static_cast<TI*>(wsbuf(c, WS_MU_LT, rp * ld * sizeof(TI)));
+  hipLaunchKernelGGL((k_pack<TI>), dim3(grid_for(rp * ld)), dim3(256), 0, c.stream, dL, r, 1, rp, r, n, ld, 1.0, lt);
+  RR_HIP(hipGetLastError());
+  TI* WL = static_cast<TI*>(wsbuf(c, WS_MU_WL, n * kdim * sizeof(TI)));
+  auto half = [&](const TI* rows, size_t ncols, TI* out, int nblk, long long sCnb) {
+    PhaseTimer pt(c, PH_GEMM1);
+    GemmArgs g;
+    g.A = tensor_data(t, prec), g.lda = ld, g.sAb = n * ld;
+    g.B = rows, g.ldb = ld;
+    g.C = out, g.ldcm = kdim, g.ldcn = 1, g.sCb = rp;
+    g.M = (int)n, g.N = (int)ncols, g.K = (int)roundup(n, 8), g.nbatch = (int)naux;
+    g.nblk = nblk, g.sCnb = sCnb;
+    GemmSpec sp;
+    sp.prec = prec, sp.out = prec, sp.cfg = gemm1_cfg(c, ncols);
+    sp.lds = gemm1_lds(c, n, ncols);
+    gemm_auto(c, sp, g);
+  };
+  half(lt, rp, WL, 0, 0);
+  TI* rt = static_cast<TI*>(wsbuf(c, WS_MU_RT, G * rp * ld * sizeof(TI)));
+  TI* WR = static_cast<TI*>(wsbuf(c, WS_MU_WR, G * n * kdim * sizeof(TI)));
+  const int slab = prec == P64 ? 16 : 32;
+  const int ns = gemm2_splits(n, kdim, slab);
+  const size_t kper = roundup((kdim + ns - 1) / ns, slab);
+  const int nsplit = (int)((kdim + kper - 1) / kper);
+  double* kpart = dK ? static_cast<double*>(wsbuf(c, WS_MU_KP, (size_t)nsplit * n * n * sizeof(double))) : nullptr;
+  double* gam = dJ ? static_cast<double*>(wsbuf(c, WS_MU_GAM, nd * naux * sizeof(double))) : nullptr;
+  double* dfac = dJ ? static_cast<double*>(wsbuf(c, WS_MU_DF, n * rp * sizeof(double))) : nullptr;
+  const size_t ng = kdim / 8, gx = (ng + 255) / 256;
+  const size_t grows = std::max<size_t>(8, (n * gx + 4095) / 4096);
+  const int gsplit = (int)((n + grows - 1) / grows);
+  double* gpart = dJ ? static_cast<double*>(wsbuf(c, WS_MU_GP, (size_t)gsplit * ng * sizeof(double))) : nullptr;
+  for (size_t g0 = 0; g0 < nd; g0 += G) {
+    const size_t gs = std::min(G, nd - g0);
+    for (size_t j = 0; j < gs; ++j)
+      hipLaunchKernelGGL((k_pack<TI>), dim3(grid_for(rp * ld)), dim3(256), 0, c.stream, dR + (g0 + j) * n * r, r, 1, rp, r,
+                         n, ld, 1.0, rt + j * rp * ld);
+    RR_HIP(hipGetLastError());
+    half(rt, gs * rp, WR, (int)rp, (long long)(n * kdim));
+    for (size_t j = 0; j < gs; ++j) {
+      const size_t k = g0 + j;
+      const TI* wr = WR + j * n * kdim;
+      if (dJ) {
+        PhaseTimer pt(c, PH_GAMMA);
+        hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(n * rp)), dim3(256), 0, c.stream, dR + k * n * r, r, 0, n, n, r, rp,
+                           1.0, dfac);
+        hipLaunchKernelGGL((k_gamma_w_partial<TI>), dim3((unsigned)gx, (unsigned)gsplit), dim3(256), 0, c.stream, WL,
+                           (const TI*)nullptr, kdim, rp, n, dfac, rp, grows, gpart);
+        hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, gpart, gsplit, kdim, rp, naux,
+                           gam + k * naux);
+        RR_HIP(hipGetLastError());
+      }
+      if (dK) {
+        {
+          PhaseTimer pt(c, PH_GEMM2);
+          GemmArgs g;
+          g.A = WL, g.lda = kdim;
+          g.B = wr, g.ldb = kdim;
+          g.C = kpart, g.ldcm = n, g.ldcn = 1, g.sCz = n * n;
+          g.M = (int)n, g.N = (int)n, g.K = (int)kdim, g.kper = (int)kper, g.nsplit = nsplit;
+          GemmSpec sp;
+          sp.prec = prec, sp.out = P64, sp.cfg = gemm2_cfg(c, n);
+          sp.lds = gemm2_lds(c, n);
+          gemm_auto(c, sp, g);
+        }
+        hipLaunchKernelGGL(k_reduce_parts, dim3(grid_for(n * n)), dim3(256), 0, c.stream, kpart, nsplit, n * n, n * n,
+                           dK + k * n * n);
+        RR_HIP(hipGetLastError());
+      }
+    }
+  }
+  if (dJ) {
+    constexpr int NK = 8;
+    const TI* B = static_cast<const TI*>(tensor_data(t, prec));
+    const size_t vn = 16 / sizeof(TI), rowlen = n * ld;
+    const size_t nvec = tri ? tri_vectors(n, ld, vn) : rowlen / vn;
+    const size_t nx = std::max<size_t>(1, (nvec + 255) / 256);
+    size_t js = std::max<size_t>(1, std::min<size_t>(32, (2048 + nx - 1) / nx));
+    js = std::min(js, std::max<size_t>(1, naux));
+    const size_t pps = (naux + js - 1) / js;
+    js = (naux + pps - 1) / pps;
+    double* jp = static_cast<double*>(wsbuf(c, WS_MU_JP, js * NK * rowlen * sizeof(double)));
+    for (size_t k0 = 0; k0 < nd; k0 += NK) {
+      const int nk = (int)std::min<size_t>(NK, nd - k0);
+      {
+        PhaseTimer pt(c, PH_COUL);
+        hipLaunchKernelGGL((k_coulomb_multi<TI, NK>), dim3((unsigned)nx, (unsigned)js), dim3(256), 0, c.stream, B, rowlen,
+                           ld, nvec, tri, naux, gam + k0 * naux, naux, nk, pps, jp);
+        RR_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_multi_jasm, dim3(grid_for((size_t)nk * n * n)), dim3(256), 0, c.stream, jp, (int)js,
+                         NK * rowlen, rowlen, ld, n, nk, tri, dJ + k0 * n * n);
+      RR_HIP(hipGetLastError());
+    }
+  }
+}
+
+void multi_jk_prec(Context& c, const Tensor& t, int prec, const double* dL, const double* dR, size_t r, size_t nd,
+                   double* dJ, double* dK) {
+  if (prec == P64) multi_jk<double>(c, t, prec, dL, dR, r, nd, dJ, dK);
+  else if (prec == P32) multi_jk<float>(c, t, prec, dL, dR, r, nd, dJ, dK);
+  else multi_jk<bf16_t>(c, t, prec, dL, dR, r, nd, dJ, dK);
+}
+
+void drop_plan(GraphPlan& p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  p = GraphPlan();
 }
 
 
@@ -1976,6 +2156,7 @@ int rr_destroy(rr_handle h) {
     if (r.nat) (void)hipFree(r.nat);
     if (r.tr) (void)hipFree(r.tr);
   }
+  rr::drop_plan(h->ctx.multi_plan);
   for (auto& kv : h->ctx.ws) (void)hipFree(kv.second.first);
   if (h->ctx.zpage) (void)hipFree(h->ctx.zpage);
   if (h->ctx.blas) rocblas_destroy_handle(h->ctx.blas);
@@ -2160,6 +2341,58 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
     rr::finish_out(c, of);
     rr::finish_out(c, oj);
     rr::finish_out(c, ok);
+  });
+}
+
+int rr_ri_jk_multi(rr_handle h, int slot, int precision, size_t n, size_t r, const double* lfac, size_t n_dens,
+                   const double* rfac, double* j_out, double* k_out, int* replayed) {
+  return guarded(h, [&](Context& c) {
+    rr::check_prec(precision);
+    rr::PrecScope prec_scope(precision);
+    rr::Tensor& t = rr::get_slot(c, slot);
+    RR_REQUIRE(t.n == n, RR_ERR_DIM, "rr_ri_jk_multi: factor dimension does not match the tensor");
+    if (replayed) *replayed = 0;
+    if (n_dens == 0 || (!j_out && !k_out)) return;
+    RR_REQUIRE(r > 0 && lfac && rfac, RR_ERR_ARG, "rr_ri_jk_multi: null or empty factors");
+    (void)rr::tensor_data(t, precision);  // the requested copy exists
+    // Inputs and outputs go through fixed staging buffers, which the captured graph reads and writes.
+    double* sl = static_cast<double*>(rr::wsbuf(c, rr::WS_MU_IN_L, n * r * sizeof(double)));
+    double* sr = static_cast<double*>(rr::wsbuf(c, rr::WS_MU_IN_R, n_dens * n * r * sizeof(double)));
+    double* sj = j_out ? static_cast<double*>(rr::wsbuf(c, rr::WS_MU_OUT_J, n_dens * n * n * sizeof(double))) : nullptr;
+    double* sk = k_out ? static_cast<double*>(rr::wsbuf(c, rr::WS_MU_OUT_K, n_dens * n * n * sizeof(double))) : nullptr;
+    RR_HIP(hipMemcpyAsync(sl, lfac, n * r * sizeof(double), hipMemcpyDefault, c.stream));
+    RR_HIP(hipMemcpyAsync(sr, rfac, n_dens * n * r * sizeof(double), hipMemcpyDefault, c.stream));
+    const std::vector<long long> key = {slot, (long long)reinterpret_cast<uintptr_t>(rr::tensor_data(t, precision)),
+                                        precision, (long long)n, (long long)r, (long long)n_dens, j_out != nullptr,
+                                        k_out != nullptr, t.symmetric};
+    rr::GraphPlan& plan = c.multi_plan;
+    if (plan.exec && plan.key == key && plan.gen == c.ws_generation) {
+      RR_HIP(hipGraphLaunch(plan.exec, c.stream));
+      if (replayed) *replayed = 1;
+    } else {
+      rr::drop_plan(plan);
+      rr::multi_jk_prec(c, t, precision, sl, sr, r, n_dens, sj, sk);  // eager: results, sizing, tuning
+      RR_HIP(hipStreamSynchronize(c.stream));
+      const unsigned long long gen = c.ws_generation;
+      const bool timing = rr::set_phase_timing(false);
+      RR_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+      try {
+        rr::multi_jk_prec(c, t, precision, sl, sr, r, n_dens, sj, sk);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(c.stream, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        rr::set_phase_timing(timing);
+        throw;
+      }
+      rr::set_phase_timing(timing);
+      RR_HIP(hipStreamEndCapture(c.stream, &plan.graph));
+      RR_REQUIRE(c.ws_generation == gen, RR_ERR_INTERNAL, "rr_ri_jk_multi: workspace reallocated during capture");
+      RR_HIP(hipGraphInstantiate(&plan.exec, plan.graph, nullptr, nullptr, 0));
+      plan.key = key, plan.gen = gen;
+    }
+    if (j_out) RR_HIP(hipMemcpyAsync(j_out, sj, n_dens * n * n * sizeof(double), hipMemcpyDefault, c.stream));
+    if (k_out) RR_HIP(hipMemcpyAsync(k_out, sk, n_dens * n * n * sizeof(double), hipMemcpyDefault, c.stream));
   });
 }
 

@@ -1,0 +1,87 @@
+"""Many densities sharing one tensor and one left factor (the sigma-product trial densities
+D_k = L R_k^T) in one graph-captured build: every J_k, K_k must equal the single-density factor build
+and the oracle, on the eager first call and on graph replays."""
+import numpy as np
+import pytest
+
+import rerdmft_amd as rr
+from rerdmft_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+SLOT = 12
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _case(naux, n, r, nd, seed, symmetric=True):
+    b = syn.ri_tensor(naux, n, 0.8, 8.0, seed, symmetric=symmetric)
+    g = syn.rng(seed + 1)
+    l = syn.orthonormal_occ(n, r, seed + 2)
+    rs = g.standard_normal((nd, n, r))
+    return b, l, rs
+
+
+@pytest.mark.parametrize("shape", [(60, 45, 7, 5), (96, 130, 40, 11)])
+def test_multi_matches_single_builds_and_oracle(ctx, oracle, shape):
+    naux, n, r, nd = shape
+    b, l, rs = _case(naux, n, r, nd, 4100 + n)
+    ctx.upload(SLOT, b, rr.COPY_FP64)
+    j, k, rep = ctx.ri_jk_multi(SLOT, l, rs)
+    assert not rep
+    j2, k2, rep2 = ctx.ri_jk_multi(SLOT, l, rs)
+    assert rep2
+    assert np.array_equal(j, j2) and np.array_equal(k, k2)
+    for i in range(nd):
+        js, ks = ctx.ri_jk_factor(SLOT, None, l, rs[i])
+        assert rel(j[i], js) < 1e-13 and rel(k[i], ks) < 1e-13
+        if i < 3:
+            jo, ko = oracle.ri_jk_factor(b.reshape(naux, -1), l @ rs[i].T, l, rs[i])
+            assert rel(j[i], jo) < 1e-12 and rel(k[i], ko) < 1e-12
+
+
+def test_multi_replay_sees_new_inputs(ctx):
+    naux, n, r, nd = 50, 40, 6, 4
+    b, l, rs = _case(naux, n, r, nd, 4200)
+    ctx.upload(SLOT, b, rr.COPY_FP64)
+    ctx.ri_jk_multi(SLOT, l, rs)
+    rs2 = rs[::-1].copy()
+    l2 = syn.orthonormal_occ(n, r, 4201)
+    j, k, rep = ctx.ri_jk_multi(SLOT, l2, rs2)
+    assert rep
+    for i in range(nd):
+        js, ks = ctx.ri_jk_factor(SLOT, None, l2, rs2[i])
+        assert rel(j[i], js) < 1e-13 and rel(k[i], ks) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [(rr.FP32, 2e-5), (rr.BF16, 3e-2)])
+def test_multi_low_precision(ctx, prec, tol):
+    naux, n, r, nd = 80, 64, 12, 9
+    b, l, rs = _case(naux, n, r, nd, 4300)
+    ctx.upload(SLOT, b, rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16)
+    j64, k64, _ = ctx.ri_jk_multi(SLOT, l, rs)
+    j, k, _ = ctx.ri_jk_multi(SLOT, l, rs, precision=prec)
+    for i in range(nd):
+        assert rel(j[i], j64[i]) < tol and rel(k[i], k64[i]) < tol
+        js, ks = ctx.ri_jk_factor(SLOT, None, l, rs[i], precision=prec)
+        assert rel(j[i], js) < 1e-6 and rel(k[i], ks) < 1e-6
+
+
+def test_multi_nonsymmetric_tensor_and_partial_outputs(ctx):
+    naux, n, r, nd = 30, 33, 5, 3
+    b, l, rs = _case(naux, n, r, nd, 4400, symmetric=False)
+    ctx.upload(SLOT, b, rr.COPY_FP64)
+    j, _, _ = ctx.ri_jk_multi(SLOT, l, rs, want_k=False)
+    _, k, _ = ctx.ri_jk_multi(SLOT, l, rs, want_j=False)
+    for i in range(nd):
+        js, ks = ctx.ri_jk_factor(SLOT, None, l, rs[i])
+        assert rel(j[i], js) < 1e-13 and rel(k[i], ks) < 1e-13
+
+
+def test_multi_rejects_mismatched_dimension(ctx):
+    b, l, rs = _case(20, 16, 3, 2, 4500)
+    ctx.upload(SLOT, b)
+    with pytest.raises(rr.GpuError):
+        ctx.ri_jk_multi(SLOT, np.zeros((17, 3)), np.zeros((2, 17, 3)))
