@@ -212,7 +212,8 @@ def main():
                 return 2.0 * naux * n * n * nocc, "TFLOP/s", "mfma"
             return 1.0 * naux * n * (n + 1) * nocc, "TFLOP/s", "mfma"
         if phase == "coulomb":
-            return 1.0 * naux * n * n * nb, "GB/s", "hbm"
+            # symmetric tensor: its unique upper triangle is read once
+            return 0.5 * naux * n * (n + 1) * nb, "GB/s", "hbm"
         if phase == "gamma":
             return 1.0 * naux * n * nocc * (2 if launches_delta else 1) * nb, "GB/s", "hbm"
         return None, None, None
