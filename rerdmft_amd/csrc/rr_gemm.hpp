@@ -764,6 +764,9 @@ __global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
     for (int j = 0; j < WN; ++j) acc[i][j] = acc_t{};
 
   for (int st = 0; st < NST - 1 && st < nstage; ++st) issue(st);
+  // Retire the kernel-argument scalar loads here: with none pending inside the loop, the compiler can
+  // wait for the fragment reads one by one (in-order LDS returns) instead of for all of them.
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt / expcnt untouched
   for (int st = 0; st < nstage; ++st) {
     if (st + NST - 2 < nstage) {  // NST-2 later stages of this wave are in flight
       if (hi)

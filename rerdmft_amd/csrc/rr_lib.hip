@@ -497,7 +497,9 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               {128, 416, 2, 1}, {256, 224, 2, 1}, {128, 208, 4, 1},
                               // lean direct-to-LDS (gemm_nt_dma2_kernel): 20-22 here, 23-24 AGPR form
                               {128, 208, 4, 1}, {256, 128, 2, 1}, {128, 128, 2, 1}, {128, 208, 4, 1},
-                              {128, 416, 2, 1}};
+                              {128, 416, 2, 1},
+                              // lean direct-to-LDS, 2 x 2 waves of 64 x 112: 25
+                              {128, 224, 2, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 // rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
@@ -538,6 +540,7 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 20: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 208, 4, 3>), dim3(grid), dim3(256), 0, s, g); return;
           case 21: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 128, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
           case 22: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 128, 2, 4>), dim3(grid), dim3(256), 0, s, g); return;
+          case 25: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 224, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
           default:
             if (launch_dma_agpr(v, std::is_same<TO, double>::value, g, grid, s)) return;
             break;

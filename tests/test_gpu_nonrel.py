@@ -177,16 +177,19 @@ def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
     ctx.upload(9, b, rr.COPY_BF16)
     outs = []
     try:
-        for cfg in [1, 2, 5, 100, 101] + list(range(102, 125)):
+        for cfg in [1, 2, 5, 100, 101] + list(range(102, 126)):
             ctx.set_tile_config(cfg)
             dj, dk = ctx.ri_jk_delta(9, c_new, c_old, precision=rr.BF16)
             j, k = ctx.ri_jk_factor(9, None, l, precision=rr.BF16)
-            outs.append((cfg, dk, k))
+            outs.append((cfg, dk, k, dj, j))
     finally:
         ctx.set_tile_config(-1)
-    for cfg, dk, k in outs[1:]:
+    for cfg, dk, k, dj, j in outs[1:]:
         assert np.array_equal(dk, outs[0][1]), f"incremental K differs for configuration {cfg}"
         assert np.array_equal(k, outs[0][2]), f"K differs for configuration {cfg}"
+        # the Coulomb contraction fused into the half-transform epilogue is tile-independent too
+        assert np.array_equal(dj, outs[0][3]), f"incremental J differs for configuration {cfg}"
+        assert np.array_equal(j, outs[0][4]), f"J differs for configuration {cfg}"
 
 
 @pytest.mark.parametrize("symmetric", [True, False])
