@@ -72,6 +72,10 @@ struct Context {
   void* zpage = nullptr;  // 256 zero bytes (source of out-of-range chunks in direct-to-LDS GEMMs)
   unsigned long long ws_generation = 0;  // bumped whenever a workspace buffer is (re)allocated
   GraphPlan multi_plan;                  // rr_ri_jk_multi
+  // Second stream for work that can overlap the exchange products (the Coulomb stream reads B while
+  // the rank-k update reads W); null in contexts without one. Default priority, like `stream`.
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 struct Error {

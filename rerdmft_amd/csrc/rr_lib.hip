@@ -783,7 +783,14 @@ void eig_device(Context& c, bool cplx, size_t n, double* a, double* w) {
 // ================================================================================================
 // Phase timing (HIP events on the library stream)
 // ================================================================================================
+// Each timed phase records a pair of events around its launches; the pairs are resolved (waited for
+// and summed) only when the totals are read (rr_timing) or too many are pending, so timing adds no
+// host synchronization to the builds and does not serialise work on different streams.
 struct PhaseTimer {
+  struct Pending {
+    int phase;
+    hipEvent_t a, b;
+  };
   Context& c;
   int phase;
   hipEvent_t a = nullptr, b = nullptr;
@@ -791,6 +798,7 @@ struct PhaseTimer {
   static int cur_prec;  // precision of the build in progress (set by the entry points)
   static double ms[24];
   static long long cnt[24];
+  static std::vector<Pending> pending;
   PhaseTimer(Context& cc, int ph) : c(cc), phase(ph + 8 * cur_prec) {
     if (!enabled) return;
     RR_HIP(hipEventCreate(&a));
@@ -799,21 +807,32 @@ struct PhaseTimer {
   }
   ~PhaseTimer() {
     if (!enabled || !a) return;
-    if (hipEventRecord(b, c.stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
-      float t = 0.f;
-      if (hipEventElapsedTime(&t, a, b) == hipSuccess) {
-        ms[phase] += t;
-        cnt[phase] += 1;
-      }
+    if (hipEventRecord(b, c.stream) == hipSuccess) {
+      pending.push_back({phase, a, b});
+      if (pending.size() > 4096) resolve();
+    } else {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+  }
+  static void resolve() {
+    for (const Pending& p : pending) {
+      float t = 0.f;
+      if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+        ms[p.phase] += t;
+        cnt[p.phase] += 1;
+      }
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending.clear();
   }
 };
 bool PhaseTimer::enabled = false;
 int PhaseTimer::cur_prec = 0;
 double PhaseTimer::ms[24] = {0};
 long long PhaseTimer::cnt[24] = {0};
+std::vector<PhaseTimer::Pending> PhaseTimer::pending;
 struct PrecScope {
   int saved;
   explicit PrecScope(int p) : saved(PhaseTimer::cur_prec) { PhaseTimer::cur_prec = (p >= 0 && p < 3) ? p : 0; }
@@ -987,7 +1006,8 @@ __global__ void k_build_dfac(const double* f0, const double* f1, size_t r, size_
 
 template <typename TI>
 ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
-                          size_t ld_l, const double* dfac = nullptr, double* gamma_out = nullptr) {
+                          size_t ld_l, const double* dfac = nullptr, double* gamma_out = nullptr,
+                          hipEvent_t gamma_done = nullptr) {
   ExchangeOut out;
   const size_t n = t.n, ld = t.ld, naux = t.naux;
   const size_t rp = roundup(std::max<size_t>(r, 1), 8);
@@ -1028,6 +1048,7 @@ ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* d
     hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux,
                        gamma_out);
     RR_HIP(hipGetLastError());
+    if (gamma_done) RR_HIP(hipEventRecord(gamma_done, c.stream));
   }
   const int slab = prec == P64 ? 16 : 32;
   const int ns = gemm2_splits(n, kdim, slab);
@@ -1072,17 +1093,50 @@ int coulomb_real_prec(Context& c, const Tensor& t, int prec, const double* dD, d
 }
 
 ExchangeOut exchange_real_prec(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
-                               size_t ld_l, const double* dfac = nullptr, double* gamma_out = nullptr) {
-  if (prec == P64) return exchange_real<double>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out);
-  if (prec == P32) return exchange_real<float>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out);
-  return exchange_real<bf16_t>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out);
+                               size_t ld_l, const double* dfac = nullptr, double* gamma_out = nullptr,
+                               hipEvent_t gamma_done = nullptr) {
+  if (prec == P64) return exchange_real<double>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out, gamma_done);
+  if (prec == P32) return exchange_real<float>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out, gamma_done);
+  return exchange_real<bf16_t>(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma_out, gamma_done);
 }
+
 
 // Coulomb stream only (gamma given): returns split partials of Jt.
 int coulomb_from_gamma(Context& c, const Tensor& t, int prec, const double* gamma, double** jpart, bool tri = false) {
   if (prec == P64) return coulomb_stream<double>(c, t, gamma, nullptr, jpart, tri);
   if (prec == P32) return coulomb_stream<float>(c, t, gamma, nullptr, jpart, tri);
   return coulomb_stream<bf16_t>(c, t, gamma, nullptr, jpart, tri);
+}
+
+// The exchange with gamma taken from the half-transform, and the Coulomb stream of that gamma run
+// on the auxiliary stream, concurrently with the rank-k update (the stream reads B, the update reads
+// W), for fp64 / fp32 operands. Falls back to one stream when the context has no auxiliary stream.
+ExchangeOut exchange_and_coulomb(Context& c, const Tensor& t, int prec, const double* dLt, const double* dRt, size_t r,
+                                 size_t ld_l, const double* dfac, double* gamma, bool want_j, bool tri, double** jpart,
+                                 int* jsplit) {
+  // bf16: the rank-k update is bound by L2 traffic that the concurrent stream slows down as much as
+  // it hides (measured); fp64 / fp32 updates are matrix-core bound and hide the stream.
+  const bool overlap = want_j && c.aux && prec != P16;
+  ExchangeOut ex = exchange_real_prec(c, t, prec, dLt, dRt, r, ld_l, dfac, gamma, overlap ? c.ev_fork : nullptr);
+  *jsplit = 0;
+  if (!want_j) return ex;
+  if (!overlap) {
+    *jsplit = coulomb_from_gamma(c, t, prec, gamma, jpart, tri);
+    return ex;
+  }
+  RR_HIP(hipStreamWaitEvent(c.aux, c.ev_fork, 0));
+  hipStream_t main = c.stream;
+  c.stream = c.aux;
+  try {
+    *jsplit = coulomb_from_gamma(c, t, prec, gamma, jpart, tri);
+  } catch (...) {
+    c.stream = main;
+    throw;
+  }
+  c.stream = main;
+  RR_HIP(hipEventRecord(c.ev_join, c.aux));
+  RR_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+  return ex;
 }
 
 void real_assemble(Context& c, const Tensor& t, double* jpart, int jsplit, const ExchangeOut* ex, double kfac,
@@ -1260,11 +1314,11 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
     RR_HIP(hipGetLastError());
     gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
   }
-  ExchangeOut ex = exchange_real_prec(c, t, prec, lt, lt + no * n, no, n, dfac, gamma);
-  ex.ksym = 1;
   double* jpart = nullptr;
   int jsplit = 0;
-  if (dJ) jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart, t.symmetric);
+  ExchangeOut ex = exchange_and_coulomb(c, t, prec, lt, lt + no * n, no, n, dfac, gamma, dJ != nullptr, t.symmetric,
+                                        &jpart, &jsplit);
+  ex.ksym = 1;
   real_assemble(c, t, jpart, jsplit, dK ? &ex : nullptr, 0.0, nullptr, nullptr, dJ, dK, t.symmetric);
 }
 
@@ -2142,6 +2196,12 @@ int rr_create(int device, rr_handle* out) {
     delete h;
     return RR_ERR_HIP;
   }
+  if (hipStreamCreateWithFlags(&h->ctx.aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ctx.ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ctx.ev_join, hipEventDisableTiming) != hipSuccess) {
+    rr_destroy(h);
+    return RR_ERR_HIP;
+  }
   *out = h;
   return RR_OK;
 }
@@ -2163,6 +2223,12 @@ int rr_destroy(rr_handle h) {
   for (auto& kv : h->ctx.ws) (void)hipFree(kv.second.first);
   if (h->ctx.zpage) (void)hipFree(h->ctx.zpage);
   if (h->ctx.blas) rocblas_destroy_handle(h->ctx.blas);
+  if (h->ctx.aux) {
+    (void)hipStreamSynchronize(h->ctx.aux);
+    (void)hipStreamDestroy(h->ctx.aux);
+  }
+  if (h->ctx.ev_fork) (void)hipEventDestroy(h->ctx.ev_fork);
+  if (h->ctx.ev_join) (void)hipEventDestroy(h->ctx.ev_join);
   (void)hipStreamDestroy(h->ctx.stream);
   delete h;
   return RR_OK;
@@ -2187,6 +2253,7 @@ int rr_set_tile_config(rr_handle h, int cfg) {
 
 int rr_timing(rr_handle h, int enable, int reset, double* ms, long long* counts, int n) {
   if (!h) return RR_ERR_ARG;
+  rr::PhaseTimer::resolve();
   for (int i = 0; i < n && i < 24; ++i) {
     if (ms) ms[i] = rr::PhaseTimer::ms[i];
     if (counts) counts[i] = rr::PhaseTimer::cnt[i];
@@ -2299,8 +2366,11 @@ int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double
         RR_HIP(hipGetLastError());
         gamma = static_cast<double*>(rr::wsbuf(c, rr::WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
       }
-      ex = rr::exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma);
-      if (fused_j) jsplit = rr::coulomb_from_gamma(c, t, precision, gamma, &jpart, t.symmetric);
+      if (fused_j)
+        ex = rr::exchange_and_coulomb(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma, true,
+                                      t.symmetric, &jpart, &jsplit);
+      else
+        ex = rr::exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma);
     }
     rr::OutBuf oj = rr::out_buf(c, rr::WS_OUT_J, j_out, n * n), ok = rr::out_buf(c, rr::WS_OUT_K, k_out, n * n);
     if (k_out && r == 0) hipLaunchKernelGGL(rr::k_zero, dim3(rr::grid_for(n * n)), dim3(256), 0, c.stream, ok.dev, n * n);
