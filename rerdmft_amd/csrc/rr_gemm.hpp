@@ -823,4 +823,169 @@ __global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
     }
 }
 
+
+// ------------------------------------------------------------------------------------------------
+// FP64 direct-to-LDS GEMM with NW waves per workgroup (two per SIMD at NW = 8, so one wave's stalls
+// are covered by the other's matrix-core work even with one workgroup per CU). A stage holds BK = 16
+// doubles (one 128-byte line) of each of the BM + BN rows; a DMA instruction moves 8 rows (lane l:
+// row l / 8, 16-byte slot l % 8 holding chunk slot ^ (row % 8), so the eight rows of a fragment
+// phase hit all 32 banks). Fragments and matrix-core steps are those of Mma<double> (lane group g
+// takes k = 4 g .. 4 g + 3 of the stage), so sums are bitwise equal to the other FP64 kernels.
+// Row clamping, compile-time waits and the zero page for a partial stage as in gemm_nt_dma2_kernel.
+// ------------------------------------------------------------------------------------------------
+template <typename TO, int BM, int BN, int WGM, int NST, int NW>
+__global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
+  using MM = Mma<double>;
+  using acc_t = typename MM::acc_t;
+  using frag = typename MM::frag;
+  constexpr int BK = 16;
+  constexpr int ROWB = 128;
+  constexpr int TI = (BM + BN) / 8;  // DMA instructions per stage
+  constexpr int STAGE = (BM + BN) * ROWB;
+  constexpr int MAXJ = (TI + NW - 1) / NW;
+  constexpr int WGN = NW / WGM;
+  constexpr int WM = BM / (16 * WGM), WN = BN / (16 * WGN);
+  constexpr int CHI = MAXJ, CLO = TI / NW;
+  static_assert(BM % 16 == 0 && BN % 16 == 0, "tiles are made of 16-row fragments");
+  static_assert(WGM * WGN == NW && WM * 16 * WGM == BM && WN * 16 * WGN == BN, "tile not divisible over the waves");
+  static_assert(NST >= 2 && CHI * (NST - 2) <= 63, "wait counts must fit vmcnt");
+
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[NST * STAGE];
+
+  const int total = g.ntiles * g.nbatch * g.nsplit;
+  int w = static_cast<int>(blockIdx.x);
+  if (g.xcd_remap) w = remap_xcd(w, total);
+  const int tile = w % g.ntiles;
+  const int rest = w / g.ntiles;
+  const int batch = rest % g.nbatch;
+  const int split = rest / g.nbatch;
+  int tm, tn;
+  if (g.tri) {
+    int t = tile;
+    int a = static_cast<int>((sqrtf(8.0f * static_cast<float>(t) + 1.0f) - 1.0f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    while (a * (a + 1) / 2 > t) --a;
+    tm = a;
+    tn = t - a * (a + 1) / 2;
+  } else {
+    tm = tile % g.tiles_m;
+    tn = tile / g.tiles_m;
+  }
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = lane & 15, grp = lane >> 4;
+  const int wm0 = (wave % WGM) * (16 * WM), wn0 = (wave / WGM) * (16 * WN);
+  const int bm0 = tm * BM, bn0 = tn * BN;
+
+  const long long kbeg = static_cast<long long>(split) * g.kper;
+  long long kend = kbeg + g.kper;
+  if (kend > g.K) kend = g.K;
+
+  const double* Ab = static_cast<const double*>(g.A) + static_cast<long long>(batch) * g.sAb;
+  const double* Bb = static_cast<const double*>(g.B) + static_cast<long long>(batch) * g.sBb;
+
+  const bool hi = wave < (TI % NW) || (TI % NW) == 0;
+  const double* src[MAXJ];
+  int kch[MAXJ];
+#pragma unroll
+  for (int j = 0; j < MAXJ; ++j) {
+    const int i = wave + NW * j;
+    const int row = 8 * min(i, TI - 1) + (lane >> 3);
+    const int c = (lane & 7) ^ (row & 7);
+    kch[j] = 2 * c;
+    if (row < BM) {
+      const int m = min(bm0 + row, g.M - 1);
+      src[j] = Ab + static_cast<long long>(m) * g.lda + 2 * c + kbeg;
+    } else {
+      const int n = min(bn0 + row - BM, g.N - 1);
+      src[j] = Bb + static_cast<long long>(n) * g.ldb + 2 * c + kbeg;
+    }
+  }
+  const unsigned lbase = __builtin_amdgcn_readfirstlane(lds_addr(lds));
+  const int nstage = static_cast<int>((kend - kbeg + BK - 1) / BK);
+  const int nfull = static_cast<int>((kend - kbeg) / BK);
+  auto issue = [&](int st) {
+    const unsigned sb = lbase + static_cast<unsigned>((st % NST) * STAGE);
+    const long long koff = static_cast<long long>(st) * BK;
+    if (st < nfull) {
+#pragma unroll
+      for (int j = 0; j < MAXJ; ++j)
+        if (j < CLO || (j < CHI && hi)) lds_dma16(src[j] + koff, sb + static_cast<unsigned>((wave + NW * j) * 1024));
+    } else {
+#pragma unroll
+      for (int j = 0; j < MAXJ; ++j)
+        if (j < CLO || (j < CHI && hi)) {
+          const void* gp = (kbeg + koff + kch[j] < kend) ? static_cast<const void*>(src[j] + koff) : g.zpage;
+          lds_dma16(gp, sb + static_cast<unsigned>((wave + NW * j) * 1024));
+        }
+    }
+  };
+
+  acc_t acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = acc_t{};
+
+  for (int st = 0; st < NST - 1 && st < nstage; ++st) issue(st);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): no scalar loads pending inside the loop
+  for (int st = 0; st < nstage; ++st) {
+    if (st + NST - 2 < nstage) {
+      if (hi)
+        wait_vm<CHI * (NST - 2)>();
+      else
+        wait_vm<CLO * (NST - 2)>();
+    } else {
+      wait_vm<0>();
+    }
+    __builtin_amdgcn_s_barrier();
+    if (st + NST - 1 < nstage) issue(st + NST - 1);
+    const unsigned char* base = lds + (st % NST) * STAGE;
+    frag fa[WM], fb[WN];
+    auto lfrag = [&](frag& f, int lrow) {
+      const unsigned char* rp = base + lrow * ROWB;
+      const int x = lrow & 7;
+      const d2 a = *reinterpret_cast<const d2*>(rp + (((2 * grp) ^ x) << 4));
+      const d2 b = *reinterpret_cast<const d2*>(rp + (((2 * grp + 1) ^ x) << 4));
+      f.v[0] = a.x, f.v[1] = a.y, f.v[2] = b.x, f.v[3] = b.y;
+    };
+#pragma unroll
+    for (int i = 0; i < WM; ++i) lfrag(fa[i], wm0 + 16 * i + r);
+#pragma unroll
+    for (int j = 0; j < WN; ++j) lfrag(fb[j], BM + wn0 + 16 * j + r);
+    MM::mma_tiles(acc, fa, fb);
+  }
+
+  const long long coff = static_cast<long long>(batch) * g.sCb + static_cast<long long>(split) * g.sCz;
+  TO* Cb = static_cast<TO*>(g.C) + coff;
+  TO* Cb2 = g.C2 ? static_cast<TO*>(g.C2) + coff : nullptr;
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int n = bn0 + wn0 + 16 * j + r;
+      if (n >= g.N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = bm0 + wm0 + 16 * i + MM::out_row(grp, e);
+        if (m >= g.M) continue;
+        int mm = m, nn = n;
+        TO* basep = Cb;
+        if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, basep = Cb2;
+        if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, basep = Cb2;
+        if (g.nblk > 0) {
+          const int b = nn / g.nblk;
+          nn -= b * g.nblk;
+          basep += static_cast<long long>(b) * g.sCnb;
+        }
+        TO* p = basep + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
+        double v = g.alpha * static_cast<double>(acc[i][j][e]);
+        if (g.beta != 0.0) v += g.beta * Conv<TO>::from(p[0]);
+        store_out<TO>(p, v);
+      }
+    }
+}
+
 }  // namespace rr

@@ -128,7 +128,7 @@ def test_tile_configurations_agree_bitwise(ctx):
     try:
         for prec in (rr.FP64, rr.FP32):
             outs = []
-            for cfg in list(range(8)) + list(range(100, 107)):
+            for cfg in list(range(8)) + list(range(100, 107)) + list(range(126, 130)):
                 ctx.set_tile_config(cfg)
                 outs.append(ctx.ri_jk_factor(6, d, l, precision=prec)[1])
             ctx.set_tile_config(-1)
@@ -208,3 +208,25 @@ def test_coulomb_upper_triangle_and_full_stream(ctx, symmetric, n):
         assert np.max(np.abs(j - ref)) <= tol * np.max(np.abs(ref)), (prec, symmetric)
         if prec == rr.FP64 and symmetric:
             assert np.array_equal(j, j.T)
+
+
+@pytest.mark.parametrize("naux,n,r", [(40, 37, 7), (64, 200, 64), (30, 300, 150)])
+def test_fp64_gemm_variants_agree_bitwise(ctx, naux, n, r):
+    """The eight-wave fp64 direct-to-LDS kernels take the same matrix-core steps in the same k order as
+    the register-direct and LDS-staged fp64 kernels: J and K must be bitwise identical."""
+    b = syn.ri_tensor(naux, n, 1.0, 10.0, 500 + n)
+    g = syn.rng(501 + n)
+    l, rf = g.standard_normal((n, r)), g.standard_normal((n, r))
+    ctx.upload(9, b, rr.COPY_FP64)
+    outs = []
+    try:
+        for cfg in [1, 2, 5, 100, 101, 126, 127, 128, 129]:
+            ctx.set_tile_config(cfg)
+            j, k = ctx.ri_jk_factor(9, None, l)          # symmetric K: triangular rank-k update
+            j2, k2 = ctx.ri_jk_factor(9, None, l, rf)    # full rank-k update
+            outs.append((cfg, j, k, j2, k2))
+    finally:
+        ctx.set_tile_config(-1)
+    for cfg, *res in outs[1:]:
+        for a, b0 in zip(res, outs[0][1:]):
+            assert np.array_equal(a, b0), f"fp64 result differs for configuration {cfg}"
