@@ -666,8 +666,8 @@ __device__ inline void wait_vm() {
   asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-template <typename TO, int BM, int BN, int WGM, int NST>
-__global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
+template <typename TO, int BM, int BN, int WGM, int NST, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void gemm_nt_dma2_kernel(GemmArgs g) {
   using MM = Mma<bf16_t>;
   using acc_t = typename MM::acc_t;
   using frag = typename MM::frag;
@@ -675,12 +675,12 @@ __global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
   constexpr int ROWB = 64;
   constexpr int TI = (BM + BN) / 16;
   constexpr int STAGE = (BM + BN) * ROWB;
-  constexpr int MAXJ = (TI + 3) / 4;
-  constexpr int WGN = 4 / WGM;
+  constexpr int MAXJ = (TI + NW - 1) / NW;
+  constexpr int WGN = NW / WGM;
   constexpr int WM = BM / (16 * WGM), WN = BN / (16 * WGN);
-  constexpr int CHI = MAXJ, CLO = TI / 4;  // DMA slots of waves 0..TI%4-1 and of the others
+  constexpr int CHI = MAXJ, CLO = TI / NW;  // DMA slots of waves 0..TI%NW-1 and of the others
   static_assert(BM % 16 == 0 && BN % 16 == 0, "tiles are made of 16-row DMA groups");
-  static_assert(WM * 16 * WGM == BM && WN * 16 * WGN == BN, "tile not divisible over the waves");
+  static_assert(WGM * WGN == NW && WM * 16 * WGM == BM && WN * 16 * WGN == BN, "tile not divisible over the waves");
   static_assert(NST >= 2 && CHI * (NST - 2) <= 63, "wait counts must fit vmcnt");
 
   __shared__ __attribute__((aligned(1024))) unsigned char lds[NST * STAGE];
@@ -719,12 +719,12 @@ __global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
   const bf16_t* Ab = static_cast<const bf16_t*>(g.A) + static_cast<long long>(batch) * g.sAb;
   const bf16_t* Bb = static_cast<const bf16_t*>(g.B) + static_cast<long long>(batch) * g.sBb;
 
-  const bool hi = wave < (TI & 3) || (TI & 3) == 0;  // this wave holds CHI slots (else CLO)
+  const bool hi = wave < (TI % NW) || (TI % NW) == 0;  // this wave holds CHI slots (else CLO)
   const bf16_t* src[MAXJ];
   int kch[MAXJ];
 #pragma unroll
   for (int j = 0; j < MAXJ; ++j) {
-    const int i = wave + 4 * j;
+    const int i = wave + NW * j;
     const int row = 16 * min(i, TI - 1) + (lane >> 2);
     const int trow = row < BM ? row : row - BM;
     const int c = (lane & 3) ^ ((trow >> 1) & 3);
@@ -746,13 +746,13 @@ __global__ __launch_bounds__(256) void gemm_nt_dma2_kernel(GemmArgs g) {
     if (st < nfull) {
 #pragma unroll
       for (int j = 0; j < MAXJ; ++j)
-        if (j < CLO || (j < CHI && hi)) lds_dma16(src[j] + koff, sb + static_cast<unsigned>((wave + 4 * j) * 1024));
+        if (j < CLO || (j < CHI && hi)) lds_dma16(src[j] + koff, sb + static_cast<unsigned>((wave + NW * j) * 1024));
     } else {
 #pragma unroll
       for (int j = 0; j < MAXJ; ++j)
         if (j < CLO || (j < CHI && hi)) {
           const void* gp = (kbeg + koff + kch[j] < kend) ? static_cast<const void*>(src[j] + koff) : g.zpage;
-          lds_dma16(gp, sb + static_cast<unsigned>((wave + 4 * j) * 1024));
+          lds_dma16(gp, sb + static_cast<unsigned>((wave + NW * j) * 1024));
         }
     }
   };

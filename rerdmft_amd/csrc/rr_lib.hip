@@ -502,7 +502,9 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               // lean direct-to-LDS, 2 x 2 waves of 64 x 112: 25
                               {128, 224, 2, 1},
                               // fp64 direct-to-LDS with eight waves: 26-29
-                              {128, 208, 8, 0, 1}, {128, 208, 8, 0, 1}, {128, 128, 4, 0, 1}, {128, 224, 4, 0, 1}};
+                              {128, 208, 8, 0, 1}, {128, 208, 8, 0, 1}, {128, 128, 4, 0, 1}, {128, 224, 4, 0, 1},
+                              // lean bf16 direct-to-LDS with eight waves: 30-33
+                              {256, 208, 8, 1}, {256, 208, 8, 1}, {256, 224, 4, 1}, {256, 224, 4, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 // rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
@@ -553,6 +555,10 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 21: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 128, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
           case 22: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 128, 2, 4>), dim3(grid), dim3(256), 0, s, g); return;
           case 25: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 224, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
+          case 30: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 208, 8, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 31: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 208, 8, 4, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 32: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 224, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 33: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 224, 4, 4, 8>), dim3(grid), dim3(512), 0, s, g); return;
           default:
             if (launch_dma_agpr(v, std::is_same<TO, double>::value, g, grid, s)) return;
             break;
