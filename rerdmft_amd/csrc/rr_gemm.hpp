@@ -825,27 +825,30 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma2_kernel(GemmArgs g) {
 
 
 // ------------------------------------------------------------------------------------------------
-// FP64 direct-to-LDS GEMM with NW waves per workgroup (two per SIMD at NW = 8, so one wave's stalls
-// are covered by the other's matrix-core work even with one workgroup per CU). A stage holds BK = 16
-// doubles (one 128-byte line) of each of the BM + BN rows; a DMA instruction moves 8 rows (lane l:
-// row l / 8, 16-byte slot l % 8 holding chunk slot ^ (row % 8), so the eight rows of a fragment
-// phase hit all 32 banks). Fragments and matrix-core steps are those of Mma<double> (lane group g
-// takes k = 4 g .. 4 g + 3 of the stage), so sums are bitwise equal to the other FP64 kernels.
+// FP64 / FP32 direct-to-LDS GEMM with NW waves per workgroup (two per SIMD at NW = 8, so one wave's
+// stalls are covered by the other's matrix-core work even with one workgroup per CU). A stage holds
+// one 128-byte line (16 doubles or 32 floats) of each of the BM + BN rows; a DMA instruction moves 8
+// rows (lane l: row l / 8, 16-byte slot l % 8 holding chunk slot ^ (row % 8), so the eight rows of a
+// fragment phase hit all 32 banks). Fragments and matrix-core steps are those of Mma<TI> (lane group
+// g takes the 32 bytes 32 g .. 32 g + 31 of the stage), so sums are bitwise equal to the other
+// kernels of the operand type.
 // Row clamping, compile-time waits and the zero page for a partial stage as in gemm_nt_dma2_kernel.
 // ------------------------------------------------------------------------------------------------
-template <typename TO, int BM, int BN, int WGM, int NST, int NW>
+template <typename TI, typename TO, int BM, int BN, int WGM, int NST, int NW>
 __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
-  using MM = Mma<double>;
+  static_assert(sizeof(TI) == 8 || sizeof(TI) == 4, "fp64 or fp32 operands");
+  using MM = Mma<TI>;
   using acc_t = typename MM::acc_t;
   using frag = typename MM::frag;
-  constexpr int BK = 16;
+  constexpr int BK = 128 / sizeof(TI);  // elements per staged row
+  constexpr int VE = 16 / sizeof(TI);    // elements per 16-byte chunk
   constexpr int ROWB = 128;
-  constexpr int TI = (BM + BN) / 8;  // DMA instructions per stage
+  constexpr int NDMA = (BM + BN) / 8;  // DMA instructions per stage
   constexpr int STAGE = (BM + BN) * ROWB;
-  constexpr int MAXJ = (TI + NW - 1) / NW;
+  constexpr int MAXJ = (NDMA + NW - 1) / NW;
   constexpr int WGN = NW / WGM;
   constexpr int WM = BM / (16 * WGM), WN = BN / (16 * WGN);
-  constexpr int CHI = MAXJ, CLO = TI / NW;
+  constexpr int CHI = MAXJ, CLO = NDMA / NW;
   static_assert(BM % 16 == 0 && BN % 16 == 0, "tiles are made of 16-row fragments");
   static_assert(WGM * WGN == NW && WM * 16 * WGM == BM && WN * 16 * WGN == BN, "tile not divisible over the waves");
   static_assert(NST >= 2 && CHI * (NST - 2) <= 63, "wait counts must fit vmcnt");
@@ -883,24 +886,24 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
   long long kend = kbeg + g.kper;
   if (kend > g.K) kend = g.K;
 
-  const double* Ab = static_cast<const double*>(g.A) + static_cast<long long>(batch) * g.sAb;
-  const double* Bb = static_cast<const double*>(g.B) + static_cast<long long>(batch) * g.sBb;
+  const TI* Ab = static_cast<const TI*>(g.A) + static_cast<long long>(batch) * g.sAb;
+  const TI* Bb = static_cast<const TI*>(g.B) + static_cast<long long>(batch) * g.sBb;
 
-  const bool hi = wave < (TI % NW) || (TI % NW) == 0;
-  const double* src[MAXJ];
+  const bool hi = wave < (NDMA % NW) || (NDMA % NW) == 0;
+  const TI* src[MAXJ];
   int kch[MAXJ];
 #pragma unroll
   for (int j = 0; j < MAXJ; ++j) {
     const int i = wave + NW * j;
-    const int row = 8 * min(i, TI - 1) + (lane >> 3);
+    const int row = 8 * min(i, NDMA - 1) + (lane >> 3);
     const int c = (lane & 7) ^ (row & 7);
-    kch[j] = 2 * c;
+    kch[j] = VE * c;
     if (row < BM) {
       const int m = min(bm0 + row, g.M - 1);
-      src[j] = Ab + static_cast<long long>(m) * g.lda + 2 * c + kbeg;
+      src[j] = Ab + static_cast<long long>(m) * g.lda + VE * c + kbeg;
     } else {
       const int n = min(bn0 + row - BM, g.N - 1);
-      src[j] = Bb + static_cast<long long>(n) * g.ldb + 2 * c + kbeg;
+      src[j] = Bb + static_cast<long long>(n) * g.ldb + VE * c + kbeg;
     }
   }
   const unsigned lbase = __builtin_amdgcn_readfirstlane(lds_addr(lds));
@@ -947,9 +950,16 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
     auto lfrag = [&](frag& f, int lrow) {
       const unsigned char* rp = base + lrow * ROWB;
       const int x = lrow & 7;
-      const d2 a = *reinterpret_cast<const d2*>(rp + (((2 * grp) ^ x) << 4));
-      const d2 b = *reinterpret_cast<const d2*>(rp + (((2 * grp + 1) ^ x) << 4));
-      f.v[0] = a.x, f.v[1] = a.y, f.v[2] = b.x, f.v[3] = b.y;
+      if constexpr (sizeof(TI) == 8) {
+        const d2 a = *reinterpret_cast<const d2*>(rp + (((2 * grp) ^ x) << 4));
+        const d2 b = *reinterpret_cast<const d2*>(rp + (((2 * grp + 1) ^ x) << 4));
+        f.v[0] = a.x, f.v[1] = a.y, f.v[2] = b.x, f.v[3] = b.y;
+      } else {
+        const f4 a = *reinterpret_cast<const f4*>(rp + (((2 * grp) ^ x) << 4));
+        const f4 b = *reinterpret_cast<const f4*>(rp + (((2 * grp + 1) ^ x) << 4));
+        f.v[0] = a.x, f.v[1] = a.y, f.v[2] = a.z, f.v[3] = a.w;
+        f.v[4] = b.x, f.v[5] = b.y, f.v[6] = b.z, f.v[7] = b.w;
+      }
     };
 #pragma unroll
     for (int i = 0; i < WM; ++i) lfrag(fa[i], wm0 + 16 * i + r);

@@ -486,7 +486,7 @@ void launch_gemm_cfg(int cfg, const GemmArgs& g, int grid, hipStream_t s) {
 struct LdsCfg {
   int bm, bn, wgm;
   int bf16_only;  // single-buffered, higher-occupancy variants built for bf16 operands only
-  int fp64_only;  // eight-wave fp64 direct-to-LDS variants
+  int fp64_only;  // eight-wave fp64 / fp32 direct-to-LDS variants (not bf16)
 };
 static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 0}, {64, 208, 4, 0},
                               {64, 224, 2, 0},  {256, 208, 4, 0}, {128, 224, 2, 0}, {128, 208, 4, 1},
@@ -501,7 +501,7 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               {128, 416, 2, 1},
                               // lean direct-to-LDS, 2 x 2 waves of 64 x 112: 25
                               {128, 224, 2, 1},
-                              // fp64 direct-to-LDS with eight waves: 26-29
+                              // fp64 / fp32 direct-to-LDS with eight waves: 26-29
                               {128, 208, 8, 0, 1}, {128, 208, 8, 0, 1}, {128, 128, 4, 0, 1}, {128, 224, 4, 0, 1},
                               // lean bf16 direct-to-LDS with eight waves: 30-33
                               {256, 208, 8, 1}, {256, 208, 8, 1}, {256, 224, 4, 1}, {256, 224, 4, 1}};
@@ -530,12 +530,12 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           default: break;
         }
       }
-      if constexpr (std::is_same<TI, double>::value) {
+      if constexpr (std::is_same<TI, double>::value || std::is_same<TI, float>::value) {
         switch (v) {
-          case 26: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TO, 128, 208, 8, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
-          case 27: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TO, 128, 208, 8, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
-          case 28: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TO, 128, 128, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
-          case 29: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TO, 128, 224, 4, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 26: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 208, 8, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 27: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 208, 8, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 28: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 128, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 29: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 224, 4, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
           default: break;
         }
       }
@@ -631,7 +631,7 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
     if (c.lds_override >= 0 && !sp.cplx && g.kbatch == 0) {
       const LdsCfg& v = kLds[c.lds_override];
       const bool fp64_wide = sp.prec == P64 && (c.lds_override == 2 || c.lds_override == 5 || c.lds_override == 6);
-      const bool unavailable = (v.bf16_only && sp.prec != P16) || (v.fp64_only && sp.prec != P64);
+      const bool unavailable = (v.bf16_only && sp.prec != P16) || (v.fp64_only && sp.prec == P16);
       if ((!g.tri || v.bm == v.bn) && !fp64_wide && !unavailable) sp.lds = c.lds_override;
     }
     if (c.lds_override == -2) sp.lds = -1;
@@ -661,7 +661,7 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
       if (g.tri && kLds[v].bm != kLds[v].bn) continue;
       if (sp.prec == P64 && (v == 2 || v == 5 || v == 6)) continue;  // these exceed the fp64 register budget
       if (kLds[v].bf16_only && sp.prec != P16) continue;
-      if (kLds[v].fp64_only && sp.prec != P64) continue;
+      if (kLds[v].fp64_only && sp.prec == P16) continue;
       cand.push_back({1, v});
     }
   hipEvent_t e0, e1;
