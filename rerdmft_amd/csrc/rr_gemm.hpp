@@ -836,7 +836,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma2_kernel(GemmArgs g) {
 // ------------------------------------------------------------------------------------------------
 template <typename TI, typename TO, int BM, int BN, int WGM, int NST, int NW>
 __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
-  static_assert(sizeof(TI) == 8 || sizeof(TI) == 4, "fp64 or fp32 operands");
+  constexpr bool B16 = std::is_same<TI, bf16_t>::value;  // two 32-k matrix-core steps per stage
   using MM = Mma<TI>;
   using acc_t = typename MM::acc_t;
   using frag = typename MM::frag;
@@ -947,6 +947,26 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
     if (st + NST - 1 < nstage) issue(st + NST - 1);
     const unsigned char* base = lds + (st % NST) * STAGE;
     frag fa[WM], fb[WN];
+    if constexpr (B16) {
+      // step h reads chunk 4 h + grp: lane group g holds k = 32 h + 8 g .. + 7, the k order of the
+      // other bf16 kernels
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        frag ga[WM], gb[WN];
+#pragma unroll
+        for (int i = 0; i < WM; ++i) {
+          const int lrow = wm0 + 16 * i + r;
+          MM::load(ga[i], reinterpret_cast<const TI*>(base + lrow * ROWB + (((4 * h + grp) ^ (lrow & 7)) << 4)));
+        }
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+          const int lrow = BM + wn0 + 16 * j + r;
+          MM::load(gb[j], reinterpret_cast<const TI*>(base + lrow * ROWB + (((4 * h + grp) ^ (lrow & 7)) << 4)));
+        }
+        MM::mma_tiles(acc, ga, gb);
+      }
+      continue;
+    }
     auto lfrag = [&](frag& f, int lrow) {
       const unsigned char* rp = base + lrow * ROWB;
       const int x = lrow & 7;
@@ -954,7 +974,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
         const d2 a = *reinterpret_cast<const d2*>(rp + (((2 * grp) ^ x) << 4));
         const d2 b = *reinterpret_cast<const d2*>(rp + (((2 * grp + 1) ^ x) << 4));
         f.v[0] = a.x, f.v[1] = a.y, f.v[2] = b.x, f.v[3] = b.y;
-      } else {
+      } else if constexpr (sizeof(TI) == 4) {
         const f4 a = *reinterpret_cast<const f4*>(rp + (((2 * grp) ^ x) << 4));
         const f4 b = *reinterpret_cast<const f4*>(rp + (((2 * grp + 1) ^ x) << 4));
         f.v[0] = a.x, f.v[1] = a.y, f.v[2] = a.z, f.v[3] = a.w;

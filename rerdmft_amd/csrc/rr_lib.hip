@@ -504,7 +504,9 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               // fp64 / fp32 direct-to-LDS with eight waves: 26-29
                               {128, 208, 8, 0, 1}, {128, 208, 8, 0, 1}, {128, 128, 4, 0, 1}, {128, 224, 4, 0, 1},
                               // lean bf16 direct-to-LDS with eight waves: 30-33
-                              {256, 208, 8, 1}, {256, 208, 8, 1}, {256, 224, 4, 1}, {256, 224, 4, 1}};
+                              {256, 208, 8, 1}, {256, 208, 8, 1}, {256, 224, 4, 1}, {256, 224, 4, 1},
+                              // bf16 through the eight-wave full-line kernel (two matrix-core steps per stage): 34-36
+                              {256, 208, 8, 1}, {256, 224, 4, 1}, {128, 208, 8, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 // rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
@@ -559,6 +561,9 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 31: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 208, 8, 4, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 32: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 224, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 33: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 224, 4, 4, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 34: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 256, 208, 8, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 35: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 256, 224, 4, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 36: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 208, 8, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
           default:
             if (launch_dma_agpr(v, std::is_same<TO, double>::value, g, grid, s)) return;
             break;
@@ -667,9 +672,7 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
   hipEvent_t e0, e1;
   RR_HIP(hipEventCreate(&e0));
   RR_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  std::pair<int, int> pick = {sp.cfg, sp.lds};
-  for (auto& cd : cand) {
+  auto time_one = [&](const std::pair<int, int>& cd) {
     GemmSpec t = sp;
     t.cfg = cd.first, t.lds = cd.second;
     RR_HIP(hipEventRecord(e0, c.stream));
@@ -678,7 +681,18 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
     RR_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     RR_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) best = ms, pick = cd;
+    return ms;
+  };
+  // One timing of every candidate, then a second one of those within 15% of the fastest; each
+  // candidate is ranked by its best time (single timings of equal kernels differ by a few percent).
+  std::vector<float> tms(cand.size());
+  float first = 1e30f;
+  for (size_t i = 0; i < cand.size(); ++i) first = std::min(first, tms[i] = time_one(cand[i]));
+  float best = 1e30f;
+  std::pair<int, int> pick = {sp.cfg, sp.lds};
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (tms[i] <= 1.15f * first) tms[i] = std::min(tms[i], time_one(cand[i]));
+    if (tms[i] < best) best = tms[i], pick = cand[i];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
