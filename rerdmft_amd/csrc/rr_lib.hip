@@ -1013,6 +1013,60 @@ __global__ __launch_bounds__(256) void k_gamma_w_partial(const TW* W, const TW* 
   }
   part[blockIdx.y * (kdim / 8) + g8] = acc;
 }
+// Eight consecutive elements (16-byte aligned) widened to double.
+template <typename TW>
+__device__ inline void load8(const TW* p, double* o) {
+  constexpr int VN = Vec16<TW>::N;
+#pragma unroll
+  for (int v = 0; v < 8 / VN; ++v) Vec16<TW>::load(p + v * VN, o + v * VN);
+}
+
+// The same partial sums as k_gamma_w_partial (identical values, same summation order per output),
+// with each thread owning one group of 8 columns i for NPP consecutive aux rows P: the 8 weights of a
+// row nu are loaded once for NPP rows of W, and every load is a 16-byte vector.
+template <typename TW, int NPP>
+__global__ __launch_bounds__(256) void k_gamma_w_blocks(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+                                                        size_t naux, const double* dfac, size_t lddf, size_t rows,
+                                                        double* part) {
+  const size_t g8n = rp / 8, nblk = (naux + NPP - 1) / NPP;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= nblk * g8n) return;
+  const size_t q = t % g8n, i0 = 8 * q, P0 = (t / g8n) * NPP;
+  const size_t nu0 = blockIdx.y * rows, nu1 = std::min(n, nu0 + rows);
+  double acc[NPP];
+#pragma unroll
+  for (int p = 0; p < NPP; ++p) acc[p] = 0.0;
+  for (size_t nu = nu0; nu < nu1; ++nu) {
+    double d[8], d2[8];
+    const double* df = dfac + nu * lddf + i0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Vec16<double>::load(df + 2 * v, d + 2 * v);
+    if (W2) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Vec16<double>::load(df + rp + 2 * v, d2 + 2 * v);
+    }
+#pragma unroll
+    for (int p = 0; p < NPP; ++p) {
+      if (P0 + p < naux) {
+        const size_t off = nu * kdim + (P0 + p) * rp + i0;
+        double w[8];
+        load8<TW>(W + off, w);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[p] += w[e] * d[e];
+        if (W2) {
+          load8<TW>(W2 + off, w);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) acc[p] += w[e] * d2[e];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NPP; ++p)
+    if (P0 + p < naux) part[blockIdx.y * (kdim / 8) + (P0 + p) * g8n + q] = acc[p];
+}
+
+
 __global__ void k_gamma_w_reduce(const double* part, int nsplit, size_t kdim, size_t rp, size_t naux, double* gamma) {
   const size_t gpp = rp / 8, ng = kdim / 8;
   for (size_t P = blockIdx.x * (size_t)blockDim.x + threadIdx.x; P < naux; P += (size_t)gridDim.x * blockDim.x) {
@@ -1021,6 +1075,35 @@ __global__ void k_gamma_w_reduce(const double* part, int nsplit, size_t kdim, si
       for (size_t q = 0; q < gpp; ++q) s += part[z * ng + P * gpp + q];
     gamma[P] = s;
   }
+}
+
+// gamma[P] = sum_(nu,i) W[nu][(P,i)] dfac[nu][i] (+ W2 with dfac columns [rp, 2rp)) on the stream.
+template <typename TW>
+void gamma_from_w(Context& c, const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n, size_t naux,
+                  const double* dfac, size_t lddf, double* gamma) {
+  if constexpr (std::is_same<TW, double>::value) {
+    // fp64 W: one 64-byte run per thread and row is already one full vector load per lane group;
+    // the per-thread layout of k_gamma_w_partial streams it faster (measured)
+    const size_t ng = kdim / 8, gx = (ng + 255) / 256;
+    const size_t rows = std::max<size_t>(8, (n * gx + 4095) / 4096);
+    const int nsplit = (int)((n + rows - 1) / rows);
+    double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * ng * sizeof(double)));
+    hipLaunchKernelGGL((k_gamma_w_partial<TW>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W, W2, kdim,
+                       rp, n, dfac, lddf, rows, part);
+    hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux, gamma);
+    RR_HIP(hipGetLastError());
+    return;
+  }
+  constexpr int NPP = 4;
+  const size_t threads = ((naux + NPP - 1) / NPP) * (rp / 8);
+  const size_t gx = (threads + 255) / 256;
+  const size_t rows = std::max<size_t>(8, (n * gx + 4095) / 4096);  // ~4096 workgroups
+  const int nsplit = (int)((n + rows - 1) / rows);
+  double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * (kdim / 8) * sizeof(double)));
+  hipLaunchKernelGGL((k_gamma_w_blocks<TW, NPP>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W, W2,
+                     kdim, rp, n, naux, dfac, lddf, rows, part);
+  hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux, gamma);
+  RR_HIP(hipGetLastError());
 }
 
 // dfac (n x ncols) for the Coulomb contraction of a factor-form density: column block b of the
@@ -1071,16 +1154,7 @@ ExchangeOut exchange_real(Context& c, const Tensor& t, int prec, const double* d
   }
   if (gamma_out) {
     PhaseTimer pt(c, PH_GAMMA);
-    const size_t ng = kdim / 8;
-    const size_t gx = (ng + 255) / 256;
-    size_t rows = std::max<size_t>(8, (n * gx + 4095) / 4096);  // ~4096 workgroups
-    const int nsplit = (int)((n + rows - 1) / rows);
-    double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * ng * sizeof(double)));
-    hipLaunchKernelGGL((k_gamma_w_partial<TI>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W,
-                       (nfac == 2 ? W2 : nullptr), kdim, rp, n, dfac, nfac * rp, rows, part);
-    hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux,
-                       gamma_out);
-    RR_HIP(hipGetLastError());
+    gamma_from_w<TI>(c, W, nfac == 2 ? W2 : nullptr, kdim, rp, n, naux, dfac, nfac * rp, gamma_out);
     if (gamma_done) RR_HIP(hipEventRecord(gamma_done, c.stream));
   }
   const int slab = prec == P64 ? 16 : 32;
@@ -1453,10 +1527,6 @@ void multi_jk(Context& c, const Tensor& t, int prec, const double* dL, const dou
   double* kpart = dK ? static_cast<double*>(wsbuf(c, WS_MU_KP, (size_t)nsplit * n * n * sizeof(double))) : nullptr;
   double* gam = dJ ? static_cast<double*>(wsbuf(c, WS_MU_GAM, nd * naux * sizeof(double))) : nullptr;
   double* dfac = dJ ? static_cast<double*>(wsbuf(c, WS_MU_DF, n * rp * sizeof(double))) : nullptr;
-  const size_t ng = kdim / 8, gx = (ng + 255) / 256;
-  const size_t grows = std::max<size_t>(8, (n * gx + 4095) / 4096);
-  const int gsplit = (int)((n + grows - 1) / grows);
-  double* gpart = dJ ? static_cast<double*>(wsbuf(c, WS_MU_GP, (size_t)gsplit * ng * sizeof(double))) : nullptr;
   for (size_t g0 = 0; g0 < nd; g0 += G) {
     const size_t gs = std::min(G, nd - g0);
     for (size_t j = 0; j < gs; ++j)
@@ -1471,11 +1541,7 @@ void multi_jk(Context& c, const Tensor& t, int prec, const double* dL, const dou
         PhaseTimer pt(c, PH_GAMMA);
         hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(n * rp)), dim3(256), 0, c.stream, dR + k * n * r, r, 0, n, n, r, rp,
                            1.0, dfac);
-        hipLaunchKernelGGL((k_gamma_w_partial<TI>), dim3((unsigned)gx, (unsigned)gsplit), dim3(256), 0, c.stream, WL,
-                           (const TI*)nullptr, kdim, rp, n, dfac, rp, grows, gpart);
-        hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, gpart, gsplit, kdim, rp, naux,
-                           gam + k * naux);
-        RR_HIP(hipGetLastError());
+        gamma_from_w<TI>(c, WL, nullptr, kdim, rp, n, naux, dfac, rp, gam + k * naux);
       }
       if (dK) {
         {
