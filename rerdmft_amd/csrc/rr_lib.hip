@@ -506,7 +506,9 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               // lean bf16 direct-to-LDS with eight waves: 30-33
                               {256, 208, 8, 1}, {256, 208, 8, 1}, {256, 224, 4, 1}, {256, 224, 4, 1},
                               // bf16 through the eight-wave full-line kernel (two matrix-core steps per stage): 34-36
-                              {256, 208, 8, 1}, {256, 224, 4, 1}, {128, 208, 8, 1}};
+                              {256, 208, 8, 1}, {256, 224, 4, 1}, {128, 208, 8, 1},
+                              // fp64 / fp32 eight-wave square tiles for the rank-k update: 37-38
+                              {128, 128, 4, 0, 1}, {192, 192, 4, 0, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 // rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
@@ -538,6 +540,8 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 27: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 208, 8, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 28: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 128, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 29: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 224, 4, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 37: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 128, 4, 4, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 38: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 192, 192, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
           default: break;
         }
       }

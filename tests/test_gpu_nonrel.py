@@ -128,7 +128,7 @@ def test_tile_configurations_agree_bitwise(ctx):
     try:
         for prec in (rr.FP64, rr.FP32):
             outs = []
-            for cfg in list(range(8)) + list(range(100, 107)) + list(range(126, 130)):
+            for cfg in list(range(8)) + list(range(100, 107)) + list(range(126, 130)) + [137, 138]:
                 ctx.set_tile_config(cfg)
                 outs.append(ctx.ri_jk_factor(6, d, l, precision=prec)[1])
             ctx.set_tile_config(-1)
@@ -220,7 +220,7 @@ def test_fp64_gemm_variants_agree_bitwise(ctx, naux, n, r):
     ctx.upload(9, b, rr.COPY_FP64)
     outs = []
     try:
-        for cfg in [1, 2, 5, 100, 101, 126, 127, 128, 129]:
+        for cfg in [1, 2, 5, 100, 101, 126, 127, 128, 129, 137, 138]:
             ctx.set_tile_config(cfg)
             j, k = ctx.ri_jk_factor(9, None, l)          # symmetric K: triangular rank-k update
             j2, k2 = ctx.ri_jk_factor(9, None, l, rf)    # full rank-k update
