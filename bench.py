@@ -48,6 +48,8 @@ def parse():
                    help="molecules in the uneven-batch leg (0: skip); n ~ U[20,200], n_aux = 4n, n_occ = n/5")
     p.add_argument("--sigma-densities", type=int, default=64,
                    help="trial densities in the shared-tensor leg (0: skip); n = 800 (100 occ + 700 virt), n_aux = 3200")
+    p.add_argument("--relativistic", type=int, default=1,
+                   help="1: time the X2C (n=400, n_aux=2400) and 4c RKB (nL=300, nS=700, n_aux=4200) builds")
     p.add_argument("--copy-probe-gib", type=float, default=0.0,
                    help="only run a device copy of this many GiB (calibrates HBM byte counters) and exit")
     return p.parse_args()
@@ -286,6 +288,7 @@ def main():
         "setup_s": setup_s,
         "batch_uneven_molecules": batch_leg(torch, rr, ctx, a.batch_molecules) if a.batch_molecules > 0 else None,
         "sigma_trial_densities": sigma_leg(torch, rr, ctx, a.sigma_densities) if a.sigma_densities > 0 else None,
+        "relativistic": relativistic_leg(torch, rr, ctx) if a.relativistic else None,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -368,6 +371,122 @@ def sigma_leg(torch, rr, ctx, nd, reps=2, slot=14):
     # exchange flops per density: half-transform of R_k and the rank-k product (B L once per call)
     out["exchange_TFLOP_per_density"] = 2 * 2.0 * naux * n * n * no / 1e12
     ctx.release(slot)
+    return out
+
+
+def _timed(torch, fn, reps):
+    fn()  # warm-up (first call of a shape: workspace sizing, tile autotuning)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def _phases(torch, ctx, fn):
+    """Device time per phase (ms) of one call of fn."""
+    ctx.timing(enable=True, reset=True)
+    fn()
+    torch.cuda.synchronize()
+    ph = ctx.timing(enable=False, reset=True)
+    return {k: round(v[0], 3) for k, v in ph.items() if v[1]}
+
+
+def relativistic_leg(torch, rr, ctx, reps=2):
+    """X2C 2-spinor Fock build (n = 400 spatial, 800 spinors, n_aux = 2400, 80 Kramers pairs) and its
+    AO->MO transform to 800 x 800 per P; 4c RKB riFockTwoElectronDirect (nL = 300, nS = 700, one shared
+    n_aux = 4200 with the LL block non-zero on its first 1800 rows, 100 Kramers pairs). The AO tensors
+    are real, as in the reference's RI sources; tensors are generated in HBM, and densities, orbitals
+    and outputs are device arrays. Flop counts are algorithmic:
+      X2C exchange  n_aux n^2 n_occ (2 spins x 4 [real B times complex C] + 3 blocks x 8) = 32 n_aux n^2 n_occ
+      X2C MO        n_aux (8 n^2 n_mo + 8 n_mo^2 2n)   (real B legs per spin, then one complex leg)
+      RKB           n_aux n_occ (4 sum_f fs_f^2 + 8 sum_{f<=u} fs_f fs_u), fs = (nL, nL, nS, nS)
+    (real B times complex C costs 4 flops per multiply-add, complex times complex 8)."""
+    from rerdmft_amd import synthetic as syn
+    dev = torch.device("cuda:0")
+    cp = rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16
+    names = (("f64", rr.FP64), ("f32", rr.FP32), ("bf16", rr.BF16))
+    out = {}
+    # X2C
+    nl, naux, npairs, slot = 400, 2400, 80, 13
+    ctx.generate(slot, naux, nl, 1.0, 40.0, 20261203 + 2, cp)
+    q = syn.kramers_pair_occ(nl, npairs, 20261203 + 12)
+    eye = torch.eye(2 * nl, dtype=torch.complex128, device=dev)
+    c = torch.from_numpy(q).to(dev)
+    p = c @ c.conj().T
+    h = torch.zeros((2 * nl, 2 * nl), dtype=torch.complex128, device=dev)
+    f = torch.empty_like(h)
+    torch.cuda.synchronize()
+    x2c = {"n_spatial": nl, "n_aux": naux, "n_occ": 2 * npairs}
+    flops = 32.0 * naux * nl * nl * (2 * npairs)
+    ref = None
+    for pname, prec in names:
+        dt = _timed(torch, lambda: ctx.x2c_fock_ri(slot, h, eye, eye, p, precision=prec, out=f), reps)
+        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "exchange_TFLOP_per_s": flops / dt / 1e12,
+             "phase_ms": _phases(torch, ctx, lambda: ctx.x2c_fock_ri(slot, h, eye, eye, p, precision=prec, out=f))}
+        if ref is None:
+            ref = f.clone()
+        else:
+            r["max_rel_dF_vs_f64"] = float((f - ref).abs().max() / ref.abs().max())
+        x2c[pname] = r
+    nt = 2 * nl
+    g = torch.Generator(device=dev)
+    g.manual_seed(20261203 + 22)
+    u, _ = torch.linalg.qr(torch.randn((nt, nt), dtype=torch.complex128, device=dev, generator=g))
+    mo_flops = naux * (8.0 * nl * nl * nt + 8.0 * nt * nt * (2 * nl))
+    for pname, prec, odt in (("f64", rr.FP64, torch.complex128), ("f32", rr.FP32, torch.complex64)):
+        mo = torch.empty((naux, nt, nt), dtype=odt, device=dev)
+        torch.cuda.synchronize()
+        dt = _timed(torch, lambda: ctx.build_ri_mo_eri(slot, slot, nl, 0, u, precision=prec, out_precision=prec,
+                                                       out=mo), 1)
+        x2c["mo_transform_" + pname] = {"ms": 1e3 * dt, "TFLOP_per_s": mo_flops / dt / 1e12}
+        del mo
+        torch.cuda.empty_cache()
+    ctx.release(slot)
+    out["x2c"] = x2c
+    # 4c RKB
+    nl, ns, naux, nll, npairs = 300, 700, 4200, 1800, 100
+    sl, ss = 10, 11
+    gl = torch.Generator(device=dev)
+    gl.manual_seed(20261203 + 3)
+    idx = torch.arange(nl, device=dev, dtype=torch.float64)
+    dec = torch.exp(-(idx[:, None] - idx[None, :]).abs() / 30.0)
+    bll = torch.zeros((naux, nl, nl), dtype=torch.float64, device=dev)
+    z = torch.randn((nll, nl, nl), dtype=torch.float64, device=dev, generator=gl)
+    bll[:nll] = 0.5 * (z + z.transpose(1, 2)) * dec
+    del z
+    torch.cuda.synchronize()
+    ctx.upload(sl, bll, cp)
+    del bll
+    ctx.generate(ss, naux, ns, 1.0, 70.0, 20261203 + 13, cp)
+    n = 2 * nl + 2 * ns
+    no = 2 * npairs
+    cq, _ = torch.linalg.qr(torch.randn((n, no), dtype=torch.complex128, device=dev, generator=gl))
+    dens = cq @ cq.conj().T
+    occ = np.ones(no)
+    gout = torch.empty((n, n), dtype=torch.complex128, device=dev)
+    torch.cuda.synchronize()
+    rkb = {"n_large": nl, "n_small": ns, "n_aux": naux, "n_aux_large_nonzero": nll, "n_occ": no}
+    fs = (nl, nl, ns, ns)
+    per_p = no * (4.0 * sum(a * a for a in fs) + 8.0 * sum(fs[i] * fs[j] for i in range(4) for j in range(i, 4)))
+    rflops = per_p * naux
+    ref = None
+    for pname, prec in names:
+        dt = _timed(torch, lambda: ctx.ri_fock_two_electron_direct(sl, ss, nl, ns, dens, cq, occ, precision=prec,
+                                                                     out=gout), reps)
+        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "exchange_TFLOP_per_s": rflops / dt / 1e12,
+             "phase_ms": _phases(torch, ctx, lambda: ctx.ri_fock_two_electron_direct(sl, ss, nl, ns, dens, cq, occ,
+                                                                                      precision=prec, out=gout))}
+        if ref is None:
+            ref = gout.clone()
+        else:
+            r["max_rel_dG_vs_f64"] = float((gout - ref).abs().max() / ref.abs().max())
+        rkb[pname] = r
+    ctx.release(sl)
+    ctx.release(ss)
+    torch.cuda.empty_cache()
+    out["rkb"] = rkb
     return out
 
 
