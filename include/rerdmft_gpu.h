@@ -50,6 +50,15 @@ const char* rr_last_error(rr_handle h);
  * fp64 / fp32 operands only; other operand types keep their default).
  * Results do not depend on the choice (every element is summed in the same order). */
 int rr_set_tile_config(rr_handle h, int cfg);
+/* How the density-taking builds (rr_nonrel_fock_ri, rr_x2c_fock_ri, rr_rkb_fock_ri) find the
+ * occupied set, in place of the reference's eigendecomposition of X^H (S P S) X (e.g.
+ * X2C_DHF/X2C_FockMatrixRi.cpp:65-95); no counterpart in the reference. method 0 (default): spectral
+ * projector onto the eigenvalues above 1/2 by a Newton-Schulz sign iteration and a factor of the
+ * occupied part, falling back to the eigensolver when the iteration does not settle (an eigenvalue at
+ * or near 1/2); 1: eigensolver only; -1: leave unchanged. Both give the same Fock matrix (the
+ * exchange depends only on sum over occupied of occupation v v^H). The counters receive the number of
+ * recoveries each method has finished on this handle (either pointer may be NULL). */
+int rr_set_recovery_method(rr_handle h, int method, long long* n_projector, long long* n_eigensolver);
 /* Per-phase device time (HIP events on the library stream) accumulated since the last reset,
  * indexed 8 * precision + phase (n up to 24): phase 0 Coulomb gamma contraction, 1 Coulomb J
  * stream, 2 exchange half-transform GEMM, 3 exchange rank-k GEMM, 4 assembly, 5 recovery;

@@ -23,7 +23,7 @@ PRECISION_NAMES = {FP64: "f64", FP32: "f32", BF16: "bf16"}
 
 # Exported C symbols declared in include/rerdmft_gpu.h (checked by the CPU test suite).
 EXPORTED = [
-    "rr_create", "rr_destroy", "rr_last_error", "rr_set_tile_config", "rr_timing",
+    "rr_create", "rr_destroy", "rr_last_error", "rr_set_tile_config", "rr_set_recovery_method", "rr_timing",
     "rr_tensor_upload", "rr_tensor_generate", "rr_tensor_info", "rr_tensor_release",
     "rr_nonrel_fock_ri", "rr_ri_jk_factor", "rr_ri_jk_multi", "rr_ri_jk_delta", "rr_x2c_fock_ri",
     "rr_ri_fock_two_electron_direct", "rr_rkb_fock_ri", "rr_build_ri_mo_eri", "rr_ri_mo_eri_rotated",
@@ -150,6 +150,13 @@ class Context:
         self._call("rr_set_tile_config", ctypes.c_int(cfg))
 
     PHASES = ["gamma", "coulomb", "exchange_half", "exchange_rank_k", "assembly", "recovery"]
+
+    def recovery_method(self, method=-1):
+        """Select the occupied-set recovery of the density-taking builds (0 spectral projector with
+        eigensolver fallback, 1 eigensolver; -1 keep); returns (projector count, eigensolver count)."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self._call("rr_set_recovery_method", ctypes.c_int(method), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def timing(self, enable=True, reset=False, by_precision=False):
         """Per-phase device time (ms, launches) since the last reset; by_precision splits by operand type."""

@@ -76,6 +76,10 @@ struct Context {
   // the rank-k update reads W); null in contexts without one. Default priority, like `stream`.
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // Occupied-orbital recovery from a density: 0 spectral projector (eigensolver when it does not
+  // converge), 1 eigensolver only; counts of the recoveries each method finished.
+  int recovery_method = 0;
+  long long n_projector = 0, n_eigensolver = 0;
 };
 
 struct Error {

@@ -1316,8 +1316,220 @@ const double* in_buf(Context& c, int id, const double* user, size_t count) {
 // keep eigenvalues > 0.5 (the top of the ascending spectrum). Returns n_occ; on return
 // `rows` points at n_occ eigenvector rows (length nx, complex when cplx) and `occ_host` holds
 // their eigenvalues.
+// ---- spectral-projector recovery ---------------------------------------------------------------
+// out = (M - shift I) * scale over an n x n array (interleaved complex when cw = 2).
+__global__ void k_shift_scale(const double* m, size_t n, int cw, double shift, double scale, double* out) {
+  const size_t total = n * n * cw;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t e = i / cw;
+    double v = m[i];
+    if (i % cw == 0 && e / n == e % n) v -= shift;
+    out[i] = v * scale;
+  }
+}
+
+// Per-block partial sums of |T - I|^2 (sub_identity) or |T|^2 over an n x n array.
+__global__ void k_fro_partial(const double* t, size_t n, int cw, int sub_identity, double* part) {
+  __shared__ double sh[256];
+  const size_t total = n * n * cw;
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t e = i / cw;
+    double v = t[i];
+    if (sub_identity && i % cw == 0 && e / n == e % n) v -= 1.0;
+    acc += v * v;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// *trace = Re tr (I + X) / 2 (one workgroup).
+__global__ void k_half_trace(const double* x, size_t n, int cw, double* trace) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (size_t d = threadIdx.x; d < n; d += blockDim.x) acc += 0.5 * (x[(d * n + d) * cw] + 1.0);
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *trace = sh[0];
+}
+
+// X <- (I + X) / 2 in place.
+__global__ void k_projector(double* x, size_t n, int cw) {
+  const size_t total = n * n * cw;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t e = i / cw;
+    x[i] = 0.5 * (x[i] + ((i % cw == 0 && e / n == e % n) ? 1.0 : 0.0));
+  }
+}
+
+// Deterministic test matrix: uniform values in [-1, 1) from a splitmix64 hash of (seed, index).
+__global__ void k_fill_uniform(double* out, size_t count, unsigned long long seed) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    out[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+// out (cols x rows) = in^T (plain transpose) of a rows x cols array.
+__global__ void k_transpose_any(const double* in, size_t rows, size_t cols, int cw, double* out) {
+  const size_t total = rows * cols;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / cols, col = i % cols;
+    for (int p = 0; p < cw; ++p) out[(col * rows + r) * cw + p] = in[i * cw + p];
+  }
+}
+
+// Zero the strictly lower triangle of a row-major n x n array.
+__global__ void k_zero_strict_lower(double* a, size_t n, int cw) {
+  const size_t total = n * n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+    if (i % n < i / n)
+      for (int p = 0; p < cw; ++p) a[i * cw + p] = 0.0;
+}
+
+namespace {
+constexpr int kFroBlocks = 256;
+
+double fro2(Context& c, const double* t, size_t n, int cw, bool sub_identity) {
+  double* part = static_cast<double*>(wsbuf(c, WS_T4 + 800, kFroBlocks * sizeof(double)));
+  hipLaunchKernelGGL(k_fro_partial, dim3(kFroBlocks), dim3(256), 0, c.stream, t, n, cw, (int)sub_identity, part);
+  RR_HIP(hipGetLastError());
+  double h[kFroBlocks];
+  RR_HIP(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
+}
+
+// Cholesky of a row-major Hermitian r x r array in place: its upper triangle becomes R with G = R^H R
+// (the column-major lower factor of G^T = conj(G), read row-major).
+void chol_upper(Context& c, bool cplx, size_t r, double* g, rocblas_int* info) {
+  rocblas_status st = cplx ? rocsolver_zpotrf(c.blas, rocblas_fill_lower, (rocblas_int)r,
+                                              reinterpret_cast<rocblas_double_complex*>(g), (rocblas_int)r, info)
+                           : rocsolver_dpotrf(c.blas, rocblas_fill_lower, (rocblas_int)r, g, (rocblas_int)r, info);
+  RR_REQUIRE(st == rocblas_status_success, RR_ERR_SOLVER, "rocsolver Cholesky failed");
+}
+
+// Y (rows x r, row-major) <- Y R^{-1} for the upper factor R of chol_upper (column-major: R^T Y^T = Y^T).
+void solve_right_upper(Context& c, bool cplx, size_t rows, size_t r, const double* g, double* y) {
+  rocblas_status st;
+  if (cplx) {
+    const rocblas_double_complex one = {1.0, 0.0};
+    st = rocblas_ztrsm(c.blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                       (rocblas_int)r, (rocblas_int)rows, &one, reinterpret_cast<const rocblas_double_complex*>(g),
+                       (rocblas_int)r, reinterpret_cast<rocblas_double_complex*>(y), (rocblas_int)r);
+  } else {
+    const double one = 1.0;
+    st = rocblas_dtrsm(c.blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                       (rocblas_int)r, (rocblas_int)rows, &one, g, (rocblas_int)r, y, (rocblas_int)r);
+  }
+  RR_REQUIRE(st == rocblas_status_success, RR_ERR_SOLVER, "rocblas triangular solve failed");
+}
+}  // namespace
+
+// Factor of the occupied part of a Hermitian M (nx x nx) without an eigensolver:
+//   Pi = step(M - 1/2) from the Newton-Schulz sign iteration X <- X (3 - X^2) / 2 started at
+//   (M - I/2) / s, s >= ||M - I/2||_2 (s = ||(M - I/2)^8||_F^(1/8)); r = tr Pi;
+//   Q = orthonormal basis of range(Pi Omega) for a fixed pseudo-random nx x r Omega (Cholesky QR twice);
+//   weighted: F = Q R_H^H with Q^H M Q = R_H^H R_H, so F F^H = Pi M Pi = sum_{lambda > 1/2} lambda v v^H;
+//   otherwise F = Q, F F^H = Pi = sum_{lambda > 1/2} v v^H.
+// Any F with the same F F^H gives the same exchange, so F replaces the reference's sqrt(occ)-scaled
+// eigenvectors. Writes F^T (r x nx) to rows_out and returns r, or returns -1 (no result) when the
+// iteration does not settle in a fixed number of steps (an eigenvalue at or near 1/2), when tr Pi is
+// not an integer or a Cholesky factorization fails; the caller then uses the eigensolver.
+long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool weighted, double* rows_out) {
+  const int cw = cplx ? 2 : 1;
+  const size_t nn = nx * nx * cw;
+  double* X = static_cast<double*>(wsbuf(c, WS_T4 + 801, nn * sizeof(double)));
+  double* T = static_cast<double*>(wsbuf(c, WS_T4 + 802, nn * sizeof(double)));
+  // spectral-norm bound from ||A^8||_F^(1/8) with A = M - I/2 (at most nx^(1/16) above ||A||_2)
+  hipLaunchKernelGGL(k_shift_scale, dim3(grid_for(nn)), dim3(256), 0, c.stream, M, nx, cw, 0.5, 1.0, X);
+  RR_HIP(hipGetLastError());
+  dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // A^2
+  dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, T, nx, T, nx, X, nx);  // A^4
+  dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // A^8
+  const double a8 = std::sqrt(fro2(c, T, nx, cw, false));
+  if (!(a8 > 0.0) || !std::isfinite(a8)) return -1;
+  const double s = std::pow(a8, 0.125);
+  hipLaunchKernelGGL(k_shift_scale, dim3(grid_for(nn)), dim3(256), 0, c.stream, M, nx, cw, 0.5, 1.0 / s, X);
+  RR_HIP(hipGetLastError());
+  constexpr int kMaxSteps = 40;
+  bool done = false;
+  for (int it = 0; it < kMaxSteps && !done; ++it) {
+    dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // X^2
+    const double e = std::sqrt(fro2(c, T, nx, cw, true));           // ||X^2 - I||_F
+    if (!std::isfinite(e) || e > 4.0 * nx) return -1;
+    done = e < 1e-9;  // one more step leaves an error at rounding level
+    dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, T, nx, X, nx, -0.5, 1.5);  // X (3 - X^2) / 2
+  }
+  if (!done) return -1;
+  double* tr = static_cast<double*>(wsbuf(c, WS_T4 + 803, sizeof(double)));
+  hipLaunchKernelGGL(k_half_trace, dim3(1), dim3(256), 0, c.stream, X, nx, cw, tr);
+  hipLaunchKernelGGL(k_projector, dim3(grid_for(nn)), dim3(256), 0, c.stream, X, nx, cw);
+  RR_HIP(hipGetLastError());
+  double trace = 0.0;
+  RR_HIP(hipMemcpyAsync(&trace, tr, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  const double rr_ = std::nearbyint(trace);
+  if (!(std::fabs(trace - rr_) < 1e-6) || rr_ < 0.0 || rr_ > (double)nx) return -1;
+  const size_t r = (size_t)rr_;
+  if (r == 0) return 0;
+  // basis of range(Pi): Y = Pi Omega, two Cholesky-QR passes
+  double* om = static_cast<double*>(wsbuf(c, WS_T4 + 804, nx * r * cw * sizeof(double)));
+  double* Y = static_cast<double*>(wsbuf(c, WS_T4 + 805, nx * r * cw * sizeof(double)));
+  double* G = static_cast<double*>(wsbuf(c, WS_T4 + 806, r * r * cw * sizeof(double)));
+  rocblas_int* info = static_cast<rocblas_int*>(wsbuf(c, WS_T4 + 807, 4 * sizeof(rocblas_int)));
+  RR_HIP(hipMemsetAsync(info, 0, 4 * sizeof(rocblas_int), c.stream));
+  hipLaunchKernelGGL(k_fill_uniform, dim3(grid_for(nx * r * cw)), dim3(256), 0, c.stream, om, nx * r * cw,
+                     0x5EEDF00DULL + nx);
+  RR_HIP(hipGetLastError());
+  dgemm_small(c, cplx, 'N', 'N', nx, r, nx, X, nx, om, r, Y, r);
+  const char ct = cplx ? 'C' : 'T';
+  for (int pass = 0; pass < 2; ++pass) {
+    dgemm_small(c, cplx, ct, 'N', r, r, nx, Y, r, Y, r, G, r);  // Y^H Y
+    chol_upper(c, cplx, r, G, info + pass);
+    solve_right_upper(c, cplx, nx, r, G, Y);
+  }
+  double* F = Y;
+  if (weighted) {
+    dgemm_small(c, cplx, 'N', 'N', nx, r, nx, M, nx, Y, r, om, r);  // M Q
+    dgemm_small(c, cplx, ct, 'N', r, r, nx, Y, r, om, r, G, r);     // H = Q^H M Q
+    chol_upper(c, cplx, r, G, info + 2);
+    hipLaunchKernelGGL(k_zero_strict_lower, dim3(grid_for(r * r)), dim3(256), 0, c.stream, G, r, cw);
+    RR_HIP(hipGetLastError());
+    dgemm_small(c, cplx, 'N', ct, nx, r, r, Y, r, G, r, om, r);  // F = Q R_H^H
+    F = om;
+  }
+  rocblas_int hinfo[4] = {0, 0, 0, 0};
+  RR_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  if (hinfo[0] || hinfo[1] || hinfo[2]) return -1;
+  hipLaunchKernelGGL(k_transpose_any, dim3(grid_for(nx * r)), dim3(256), 0, c.stream, F, nx, r, cw, rows_out);
+  RR_HIP(hipGetLastError());
+  return (long)r;
+}
+
+// Occupied set of a density (X2C_FockMatrixRi.cpp:65-95, NonRelFockMatrixRi.cpp:51-69,
+// UkbFockMatrixRi.cpp:185-211): M = scale X^H (S P S) X; the occupied vectors are the eigenvectors of M
+// with eigenvalue above 1/2. On return *rows holds n_occ rows of length nx (row i = occupied vector i)
+// and occ_host their weights: eigenvalues of M (eigensolver), or ones for the rows of a factor F with
+// F F^H = sum lambda v v^H (projector path, `weighted`), or ones with F F^H = sum v v^H (unweighted:
+// callers that use only the vectors, such as the NON_REL build with its fixed sqrt 2).
 size_t recover_occupied(Context& c, bool cplx, size_t n, size_t nx, const double* dS, const double* dP, const double* dX,
-                        double scale, const double** rows, std::vector<double>& occ_host) {
+                        double scale, const double** rows, std::vector<double>& occ_host, bool weighted) {
   PhaseTimer pt(c, PH_RECOVER);
   const int cw = cplx ? 2 : 1;
   const char T = cplx ? 'C' : 'T';
@@ -1329,11 +1541,22 @@ size_t recover_occupied(Context& c, bool cplx, size_t n, size_t nx, const double
   dgemm_small(c, cplx, 'N', 'N', n, n, n, t1, n, dS, n, t2, n, scale);          // S P S * scale
   dgemm_small(c, cplx, 'N', 'N', n, nx, n, t2, n, dX, nx, t3, nx);              // (SPS) X
   dgemm_small(c, cplx, T, 'N', nx, nx, n, dX, nx, t3, nx, t4, nx);              // X^H (SPS) X
+  if (nx == 0) return 0;
+  if (c.recovery_method == 0) {
+    const long r = projector_factor(c, cplx, nx, t4, weighted, t3);
+    if (r >= 0) {
+      ++c.n_projector;
+      occ_host.assign((size_t)r, 1.0);
+      *rows = t3;
+      return (size_t)r;
+    }
+  }
   double* w = static_cast<double*>(wsbuf(c, WS_EIGW, std::max<size_t>(nx, 1) * sizeof(double)));
   eig_device(c, cplx, nx, t4, w);
   std::vector<double> wh(nx);
   RR_HIP(hipMemcpyAsync(wh.data(), w, nx * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));
+  ++c.n_eigensolver;
   size_t first = nx;
   while (first > 0 && wh[first - 1] > 0.5) --first;
   occ_host.assign(wh.begin() + first, wh.end());
@@ -2354,6 +2577,18 @@ int rr_set_tile_config(rr_handle h, int cfg) {
   return RR_OK;
 }
 
+int rr_set_recovery_method(rr_handle h, int method, long long* n_projector, long long* n_eigensolver) {
+  if (!h) return RR_ERR_ARG;
+  if (method != -1 && method != 0 && method != 1) {
+    h->ctx.err = "recovery method out of range";
+    return RR_ERR_ARG;
+  }
+  if (method >= 0) h->ctx.recovery_method = method;
+  if (n_projector) *n_projector = h->ctx.n_projector;
+  if (n_eigensolver) *n_eigensolver = h->ctx.n_eigensolver;
+  return RR_OK;
+}
+
 int rr_timing(rr_handle h, int enable, int reset, double* ms, long long* counts, int n) {
   if (!h) return RR_ERR_ARG;
   rr::PhaseTimer::resolve();
@@ -2502,7 +2737,7 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
     // Recovery: D = P/2, C_occ = X Y_occ; c_t = sqrt(2) C_occ^T = sqrt(2) Y_occ^T X^T.
     const double* rows = nullptr;
     std::vector<double> occ;
-    const size_t n_occ = rr::recover_occupied(c, false, nl, nx, dS, dP, dX, 0.5, &rows, occ);
+    const size_t n_occ = rr::recover_occupied(c, false, nl, nx, dS, dP, dX, 0.5, &rows, occ, false);
     rr::ExchangeOut ex;
     if (n_occ > 0) {
       double* lt = static_cast<double*>(rr::wsbuf(c, rr::WS_COCC, n_occ * nl * sizeof(double)));
@@ -2611,7 +2846,7 @@ int rr_x2c_fock_ri(rr_handle h, int slot, int precision, size_t nl, const double
     // recovery (X2C_FockMatrixRi.cpp:65-95): occupations = eigenvalues above 0.5
     const double* rows = nullptr;
     std::vector<double> occ;
-    const size_t n_occ = rr::recover_occupied(c, true, n, nx, dS, dP, dX, 1.0, &rows, occ);
+    const size_t n_occ = rr::recover_occupied(c, true, n, nx, dS, dP, dX, 1.0, &rows, occ, true);
     rr::ExchangeOut ex;
     if (n_occ > 0) {
       double* ct = static_cast<double*>(rr::wsbuf(c, rr::WS_COCC, 2 * n_occ * n * sizeof(double)));
@@ -2678,7 +2913,7 @@ int rr_rkb_fock_ri(rr_handle h, int slot_ll, int slot_ss, int precision, size_t 
     const double* dP = rr::in_buf(c, rr::WS_P, density, 2 * n_rkb * n_rkb);
     const double* rows = nullptr;
     std::vector<double> occ;
-    const size_t n_occ = rr::recover_occupied(c, true, n_rkb, nx, dS, dP, dX, 1.0, &rows, occ);
+    const size_t n_occ = rr::recover_occupied(c, true, n_rkb, nx, dS, dP, dX, 1.0, &rows, occ, true);
     double* ct_ukb = nullptr;
     double* docc = nullptr;
     if (n_occ) {
