@@ -398,11 +398,13 @@ def relativistic_leg(torch, rr, ctx, reps=2):
     AO->MO transform to 800 x 800 per P; 4c RKB riFockTwoElectronDirect (nL = 300, nS = 700, one shared
     n_aux = 4200 with the LL block non-zero on its first 1800 rows, 100 Kramers pairs). The AO tensors
     are real, as in the reference's RI sources; tensors are generated in HBM, and densities, orbitals
-    and outputs are device arrays. Flop counts are algorithmic:
-      X2C exchange  n_aux n^2 n_occ (2 spins x 4 [real B times complex C] + 3 blocks x 8) = 32 n_aux n^2 n_occ
+    and outputs are device arrays. "reference_gemm_TFLOP_per_s" divides the flops of the reference's
+    own GEMM calls for the build (real B times complex C: 4 flops per multiply-add, complex times
+    complex: 8; full Hermitian blocks, aux rows that are zero left out) by the build time:
+      X2C exchange  n_aux n^2 n_occ (2 spins x 4 + 3 blocks x 8) = 32 n_aux n^2 n_occ
       X2C MO        n_aux (8 n^2 n_mo + 8 n_mo^2 2n)   (real B legs per spin, then one complex leg)
-      RKB           n_aux n_occ (4 sum_f fs_f^2 + 8 sum_{f<=u} fs_f fs_u), fs = (nL, nL, nS, nS)
-    (real B times complex C costs 4 flops per multiply-add, complex times complex 8)."""
+      RKB           sum_P n_occ (4 sum_f fs_f^2 + 8 sum_{f<=u} fs_f fs_u) over the flavors f of
+                    (nL, nL, nS, nS) whose tensor row P is non-zero."""
     from rerdmft_amd import synthetic as syn
     dev = torch.device("cuda:0")
     cp = rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16
@@ -423,7 +425,7 @@ def relativistic_leg(torch, rr, ctx, reps=2):
     ref = None
     for pname, prec in names:
         dt = _timed(torch, lambda: ctx.x2c_fock_ri(slot, h, eye, eye, p, precision=prec, out=f), reps)
-        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "exchange_TFLOP_per_s": flops / dt / 1e12,
+        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "reference_gemm_TFLOP_per_s": flops / dt / 1e12,
              "phase_ms": _phases(torch, ctx, lambda: ctx.x2c_fock_ri(slot, h, eye, eye, p, precision=prec, out=f))}
         if ref is None:
             ref = f.clone()
@@ -440,7 +442,7 @@ def relativistic_leg(torch, rr, ctx, reps=2):
         torch.cuda.synchronize()
         dt = _timed(torch, lambda: ctx.build_ri_mo_eri(slot, slot, nl, 0, u, precision=prec, out_precision=prec,
                                                        out=mo), 1)
-        x2c["mo_transform_" + pname] = {"ms": 1e3 * dt, "TFLOP_per_s": mo_flops / dt / 1e12}
+        x2c["mo_transform_" + pname] = {"ms": 1e3 * dt, "reference_gemm_TFLOP_per_s": mo_flops / dt / 1e12}
         del mo
         torch.cuda.empty_cache()
     ctx.release(slot)
@@ -468,14 +470,15 @@ def relativistic_leg(torch, rr, ctx, reps=2):
     gout = torch.empty((n, n), dtype=torch.complex128, device=dev)
     torch.cuda.synchronize()
     rkb = {"n_large": nl, "n_small": ns, "n_aux": naux, "n_aux_large_nonzero": nll, "n_occ": no}
-    fs = (nl, nl, ns, ns)
-    per_p = no * (4.0 * sum(a * a for a in fs) + 8.0 * sum(fs[i] * fs[j] for i in range(4) for j in range(i, 4)))
-    rflops = per_p * naux
+    def per_p(fs):
+        return no * (4.0 * sum(a * a for a in fs) + 8.0 * sum(fs[i] * fs[j] for i in range(len(fs))
+                                                              for j in range(i, len(fs))))
+    rflops = nll * per_p((nl, nl, ns, ns)) + (naux - nll) * per_p((ns, ns))
     ref = None
     for pname, prec in names:
         dt = _timed(torch, lambda: ctx.ri_fock_two_electron_direct(sl, ss, nl, ns, dens, cq, occ, precision=prec,
                                                                      out=gout), reps)
-        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "exchange_TFLOP_per_s": rflops / dt / 1e12,
+        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "reference_gemm_TFLOP_per_s": rflops / dt / 1e12,
              "phase_ms": _phases(torch, ctx, lambda: ctx.ri_fock_two_electron_direct(sl, ss, nl, ns, dens, cq, occ,
                                                                                       precision=prec, out=gout))}
         if ref is None:

@@ -31,7 +31,11 @@ struct Tensor {
   float* f32 = nullptr;
   bf16_t* b16 = nullptr;
   bool symmetric = true;
+  size_t naux_nz = SIZE_MAX;  // aux rows from this index on are all zero (SIZE_MAX: not known)
 };
+
+// Aux rows that can hold non-zero values (at least 1 when the tensor has rows).
+inline size_t live_rows(const Tensor& t) { return t.naux ? std::max<size_t>(1, std::min(t.naux, t.naux_nz)) : 0; }
 
 constexpr int kSlots = 16;
 

@@ -1870,11 +1870,20 @@ ExchangeOut exchange_flavors(Context& c, int prec, const std::vector<Flavor>& fl
   const size_t kdim = naux * rp;          // complex k of the rank-k update
   const size_t ldw = 2 * kdim;            // reals per W row
   TI* W = static_cast<TI*>(wsbuf(c, WS_W, n_stack * ldw * sizeof(TI)));
+  // Aux rows past a tensor's last non-zero row add nothing: its half-transform stops there and the
+  // rank-k update runs over the common live columns for every row, then over the rest for the rows of
+  // the pair with more live rows only (a shared aux index, UkbFockMatrixRi.cpp:46-49, leaves trailing
+  // zero rows in the smaller block). W columns past a pair's live rows are never read.
+  size_t live[2] = {0, 0}, row0[2] = {0, 0}, rows[2] = {0, 0};
+  const int npairs = (int)(fl.size() / 2);
+  RR_REQUIRE(npairs >= 1 && npairs <= 2 && fl.size() % 2 == 0, RR_ERR_ARG, "exchange: one or two flavor pairs");
   // stacked row offset of each flavor = its offset (flavors are listed in output order)
   for (size_t pi = 0; pi + 1 < fl.size(); pi += 2) {
     const Flavor& f0 = fl[pi];
     const Flavor& f1 = fl[pi + 1];
     const Tensor& t = *f0.t;
+    const size_t nz = live_rows(t);
+    live[pi / 2] = nz, row0[pi / 2] = f0.off, rows[pi / 2] = f0.fs + f1.fs;
     TI* lt = static_cast<TI*>(wsbuf(c, WS_LT, 4 * rp * t.ld * sizeof(TI)));
     hipLaunchKernelGGL((k_build_ct<TI>), dim3(grid_for(4 * rp * t.ld)), dim3(256), 0, c.stream, dCt, ldct, dOcc, n_occ, rp,
                        f0.off, f1.off, f0.fs, t.ld, lt);
@@ -1885,30 +1894,51 @@ ExchangeOut exchange_flavors(Context& c, int prec, const std::vector<Flavor>& fl
     g.B = lt, g.ldb = t.ld;
     g.C = W + f0.off * ldw, g.C2 = W + f1.off * ldw, g.ncut = (int)(2 * rp);
     g.ldcm = ldw, g.ldcn = 1, g.sCb = 2 * rp;
-    g.M = (int)t.n, g.N = (int)(4 * rp), g.K = (int)roundup(t.n, 8), g.nbatch = (int)naux;
+    g.M = (int)t.n, g.N = (int)(4 * rp), g.K = (int)roundup(t.n, 8), g.nbatch = (int)nz;
     GemmSpec sp;
     sp.prec = prec, sp.out = prec, sp.cfg = gemm1_cfg(c, 4 * rp);
     sp.lds = gemm1_lds(c, t.n, 4 * rp);
     gemm_auto(c, sp, g);
   }
   const int slab = prec == P64 ? 16 : 32;
-  const int ns = gemm2_splits(n_stack, 2 * kdim, slab);
-  const size_t kper = roundup((kdim + ns - 1) / ns, slab);  // complex k per split (2*kper reals: multiple of slab)
-  const int nsplit = (int)((kdim + kper - 1) / kper);
-  double* kpart = static_cast<double*>(wsbuf(c, WS_KPART, (size_t)nsplit * n_stack * n_stack * 2 * sizeof(double)));
+  const size_t kA = (npairs == 2 ? std::min(live[0], live[1]) : live[0]) * rp;  // complex columns for all rows
+  const int big = (npairs == 2 && live[1] > live[0]) ? 1 : 0;
+  const size_t kB = npairs == 2 ? std::max(live[0], live[1]) * rp - kA : 0;    // columns for pair `big` only
+  auto split_of = [&](size_t m, size_t kd, size_t* kper) {
+    const int ns = gemm2_splits(m, 2 * kd, slab);
+    *kper = roundup((kd + ns - 1) / ns, slab);  // complex k per split (2*kper reals: multiple of slab)
+    return (int)((kd + *kper - 1) / *kper);
+  };
+  size_t kperA = 0, kperB = 0;
+  const int nsA = split_of(n_stack, kA, &kperA);
+  const int nsB = kB ? split_of(rows[big], kB, &kperB) : 0;
+  const size_t nsq = n_stack * n_stack;
+  double* kpart = static_cast<double*>(wsbuf(c, WS_KPART, (size_t)(nsA + nsB) * nsq * 2 * sizeof(double)));
   {
     PhaseTimer pt(c, PH_GEMM2);
     GemmArgs g;
     g.A = W, g.lda = kdim, g.B = W, g.ldb = kdim;
-    g.C = kpart, g.ldcm = n_stack, g.ldcn = 1, g.sCz = n_stack * n_stack;
-    g.M = (int)n_stack, g.N = (int)n_stack, g.K = (int)kdim, g.kper = (int)kper, g.nsplit = nsplit;
+    g.C = kpart, g.ldcm = n_stack, g.ldcn = 1, g.sCz = nsq;
+    g.M = (int)n_stack, g.N = (int)n_stack, g.K = (int)kA, g.kper = (int)kperA, g.nsplit = nsA;
     g.tri = 1, g.conj = 1;
     GemmSpec sp;
     sp.prec = prec, sp.out = P64, sp.cplx = true, sp.cfg = gemm2_cfg(c, n_stack);
     gemm_auto(c, sp, g);
+    if (kB) {
+      // the other rows of these partial sums are zero
+      double* pb = kpart + (size_t)nsA * nsq * 2;
+      hipLaunchKernelGGL(k_zero, dim3(grid_for((size_t)nsB * nsq * 2)), dim3(256), 0, c.stream, pb, (size_t)nsB * nsq * 2);
+      RR_HIP(hipGetLastError());
+      const TI* wb = W + row0[big] * ldw + 2 * kA;
+      g.A = wb, g.B = wb;
+      g.C = pb + 2 * (row0[big] * n_stack + row0[big]);
+      g.M = (int)rows[big], g.N = (int)rows[big], g.K = (int)kB, g.kper = (int)kperB, g.nsplit = nsB;
+      sp.cfg = gemm2_cfg(c, rows[big]);
+      gemm_auto(c, sp, g);
+    }
     out.ktile = 1;
   }
-  out.kpart = kpart, out.ksplit = nsplit, out.kstride = n_stack * n_stack;
+  out.kpart = kpart, out.ksplit = nsA + nsB, out.kstride = nsq;
   return out;
 }
 
@@ -2415,13 +2445,25 @@ void tensor_free(Tensor& t) {
   t = Tensor();
 }
 
-// flag[0] = 1 if some row block of src (rows x n x n) is not exactly symmetric.
-__global__ void k_sym_check(const double* src, size_t rows, size_t n, int* flag) {
+// flag[0] = 1 if some row block of src (rows x n x n) is not exactly symmetric; flag[1] = max(flag[1],
+// 1 + index (offset by p0) of the last row holding a non-zero value).
+__global__ void k_sym_check(const double* src, size_t rows, size_t n, size_t p0, int* flag) {
+  __shared__ int last[256];
   const size_t nn = n * n, total = rows * nn;
+  int mine = 0;
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const size_t P = idx / nn, rem = idx % nn, mu = rem / n, nu = rem % n;
-    if (nu > mu && !(src[idx] == src[P * nn + nu * n + mu])) flag[0] = 1;
+    const double v = src[idx];
+    if (nu > mu && !(v == src[P * nn + nu * n + mu])) flag[0] = 1;
+    if (v != 0.0) mine = max(mine, (int)(p0 + P + 1));
   }
+  last[threadIdx.x] = mine;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) last[threadIdx.x] = max(last[threadIdx.x], last[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && last[0] > 0) atomicMax(flag + 1, last[0]);
 }
 
 void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t n, unsigned copies) {
@@ -2441,8 +2483,8 @@ void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t
   chunk = std::min<size_t>(chunk, 65535);
   double* stage = dev ? nullptr : static_cast<double*>(wsbuf(c, WS_STAGE, chunk * rowbytes));
   // Exactly symmetric blocks (mu nu|P) = (nu mu|P) let the Coulomb stream read only the upper triangle.
-  int* asym = static_cast<int*>(wsbuf(c, WS_TMP9 + 77, sizeof(int)));
-  RR_HIP(hipMemsetAsync(asym, 0, sizeof(int), c.stream));
+  int* asym = static_cast<int*>(wsbuf(c, WS_TMP9 + 77, 2 * sizeof(int)));
+  RR_HIP(hipMemsetAsync(asym, 0, 2 * sizeof(int), c.stream));
   for (size_t p0 = 0; p0 < naux; p0 += chunk) {
     const size_t rows = std::min(chunk, naux - p0);
     const double* src = eri3 + p0 * n * n;
@@ -2454,14 +2496,15 @@ void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t
     if (t.f64) hipLaunchKernelGGL(k_store_rows<double>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f64 + p0 * n * t.ld);
     if (t.f32) hipLaunchKernelGGL(k_store_rows<float>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.f32 + p0 * n * t.ld);
     if (t.b16) hipLaunchKernelGGL(k_store_rows<bf16_t>, grid, dim3(256), 0, c.stream, src, n, t.ld, t.b16 + p0 * n * t.ld);
-    hipLaunchKernelGGL(k_sym_check, dim3(grid_for(rows * n * n)), dim3(256), 0, c.stream, src, rows, n, asym);
+    hipLaunchKernelGGL(k_sym_check, dim3(grid_for(rows * n * n)), dim3(256), 0, c.stream, src, rows, n, p0, asym);
     RR_HIP(hipGetLastError());
     if (!dev) RR_HIP(hipStreamSynchronize(c.stream));
   }
-  int h_asym = 0;
-  RR_HIP(hipMemcpyAsync(&h_asym, asym, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  int h_asym[2] = {0, 0};
+  RR_HIP(hipMemcpyAsync(h_asym, asym, sizeof(h_asym), hipMemcpyDeviceToHost, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));
-  t.symmetric = h_asym == 0;
+  t.symmetric = h_asym[0] == 0;
+  t.naux_nz = (size_t)h_asym[1];
 }
 
 // J and K of D = L L^T (L device, n x r) on the context stream with no host synchronization: the

@@ -110,13 +110,39 @@ def test_rkb_fock_ri_embedding(ctx, oracle):
     assert rel(f, fr) < 1e-11
 
 
-def test_rkb_shared_aux_with_zero_large_rows(ctx, oracle):
-    # unequal aux sets as one shared index: the LL tensor is zero beyond its own aux count
+@pytest.mark.parametrize("zero_ll,zero_ss,prec", [(20, None, rr.FP64), (None, 23, rr.FP64), (1, 35, rr.FP64),
+                                                   (20, None, rr.BF16), (None, 9, rr.FP32)])
+def test_rkb_shared_aux_with_zero_trailing_rows(ctx, oracle, zero_ll, zero_ss, prec):
+    # unequal aux sets as one shared index: a tensor is zero beyond its own aux count
     nl, ns, naux, nocc = 8, 12, 36, 5
     bll, bss, c, occ, dens = ukb_inputs(nl, ns, naux, nocc, 601)
-    bll[20:] = 0.0
-    ctx.upload(8, bll)
-    ctx.upload(9, bss)
-    g = ctx.ri_fock_two_electron_direct(8, 9, nl, ns, dens, c, occ)
+    if zero_ll is not None:
+        bll[zero_ll:] = 0.0
+    if zero_ss is not None:
+        bss[zero_ss:] = 0.0
+    cp = rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16
+    ctx.upload(8, bll, cp)
+    ctx.upload(9, bss, cp)
+    g = ctx.ri_fock_two_electron_direct(8, 9, nl, ns, dens, c, occ, precision=prec)
     gr = oracle.ri_fock_two_electron_direct(nl, ns, bll.reshape(naux, -1), bss.reshape(naux, -1), dens, c, occ)
-    assert rel(g, gr) < 1e-11
+    assert rel(g, gr) < {rr.FP64: 1e-11, rr.FP32: 1e-5, rr.BF16: 3e-2}[prec]
+    if prec == rr.FP64:
+        # the same tensors with every row live (trailing rows set to a tiny value) agree
+        bll2, bss2 = bll.copy(), bss.copy()
+        for z, b2 in ((zero_ll, bll2), (zero_ss, bss2)):
+            if z is not None:
+                b2[-1, 0, 0] = 1e-300
+        ctx.upload(8, bll2)
+        ctx.upload(9, bss2)
+        g2 = ctx.ri_fock_two_electron_direct(8, 9, nl, ns, dens, c, occ)
+        assert rel(g2, g) < 1e-13
+
+
+def test_x2c_with_zero_trailing_aux_rows(ctx, oracle):
+    nl, naux = 30, 80
+    b, s, x, p, h = x2c_inputs(nl, naux, 4, 611)
+    b[50:] = 0.0
+    ctx.upload(10, b)
+    f = ctx.x2c_fock_ri(10, h, x, s, p)
+    fr = oracle.x2c_fock_ri(b.reshape(naux, -1), x, s, h, p)
+    assert rel(f, fr) < 1e-11
