@@ -20,6 +20,14 @@ struct BatchLane {
   hipEvent_t done = nullptr;
 };
 
+// One contiguous device-to-device copy of a segmented copy launch.
+struct CopySeg {
+  const double* src;
+  double* dst;
+  unsigned long long count;
+  unsigned long long first_block;  // blocks of this segment: [first_block, next segment's first_block)
+};
+
 struct rr_batch {
   std::vector<rr::Tensor> t;
   std::vector<BatchLane*> lanes;
@@ -34,12 +42,59 @@ struct rr_batch {
   hipGraphExec_t exec = nullptr;
   std::vector<unsigned long long> generation;  // lane workspace generations at capture
   hipEvent_t fork = nullptr;
+  CopySeg* segs = nullptr;           // device table of the per-call gather / scatter copies
+  size_t seg_cap = 0;
 };
+
+// Many device-to-device copies in one launch: block b copies a 4 KiB chunk of the segment whose
+// block range holds b (found by binary search over the table's first_block column).
+__global__ __launch_bounds__(256) void k_segment_copy(const CopySeg* segs, int nseg) {
+  int lo = 0, hi = nseg;
+  const unsigned long long b = blockIdx.x;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (segs[mid].first_block <= b) lo = mid; else hi = mid;
+  }
+  const CopySeg sg = segs[lo];
+  const unsigned long long base = (b - sg.first_block) * 512;
+  for (unsigned long long i = base + threadIdx.x; i < base + 512 && i < sg.count; i += blockDim.x) sg.dst[i] = sg.src[i];
+}
 
 namespace {
 
 using rr::Context;
 using rr::Error;
+
+constexpr unsigned long long kCopyChunk = 512;  // doubles per block of k_segment_copy
+
+// Device-to-device copies (src, dst, count) in one launch; returns false if there is nothing to copy.
+bool segmented_copy(Context& c, rr_batch* b, const std::vector<CopySeg>& in) {
+  std::vector<CopySeg> segs;
+  unsigned long long blocks = 0;
+  for (const CopySeg& sg : in) {
+    if (!sg.count) continue;
+    CopySeg x = sg;
+    x.first_block = blocks;
+    blocks += (sg.count + kCopyChunk - 1) / kCopyChunk;
+    segs.push_back(x);
+  }
+  if (segs.empty()) return false;
+  RR_REQUIRE(blocks < (1ull << 31), RR_ERR_ARG, "batch: copy too large");
+  if (segs.size() > b->seg_cap) {
+    RR_HIP(hipStreamSynchronize(c.stream));
+    if (b->segs) RR_HIP(hipFree(b->segs));
+    b->segs = nullptr;
+    RR_HIP(hipMalloc(&b->segs, segs.size() * sizeof(CopySeg)));
+    b->seg_cap = segs.size();
+  }
+  // the table is read by the kernel launched right after on the same stream; the synchronous copy
+  // from pageable memory returns once the table is in device memory
+  RR_HIP(hipMemcpyAsync(b->segs, segs.data(), segs.size() * sizeof(CopySeg), hipMemcpyHostToDevice, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  hipLaunchKernelGGL(k_segment_copy, dim3((unsigned)blocks), dim3(256), 0, c.stream, b->segs, (int)segs.size());
+  RR_HIP(hipGetLastError());
+  return true;
+}
 
 void drop_graph(rr_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -150,6 +205,7 @@ int rr_batch_destroy(rr_handle h, rr_batch_t b) {
     if (b->lin) (void)hipFree(b->lin);
     if (b->jout) (void)hipFree(b->jout);
     if (b->kout) (void)hipFree(b->kout);
+    if (b->segs) (void)hipFree(b->segs);
     delete b;
   });
 }
@@ -191,16 +247,18 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
     }
     // factors into the staging buffer (host arrays are gathered into one transfer)
     std::vector<double> host;
+    std::vector<CopySeg> dcopy;
     for (size_t m = 0; m < nm; ++m) {
       const size_t cnt = b->t[m].n * r[m];
       if (!cnt) continue;
       if (rr::is_device_ptr(lfac[m])) {
-        RR_HIP(hipMemcpyAsync(b->lin + loff[m], lfac[m], cnt * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        dcopy.push_back({lfac[m], b->lin + loff[m], cnt, 0});
       } else {
         if (host.empty()) host.assign(ltot, 0.0);
         std::copy(lfac[m], lfac[m] + cnt, host.begin() + loff[m]);
       }
     }
+    segmented_copy(c, b, dcopy);
     if (!host.empty()) {
       // device-pointer factors were copied in place; copy the host part around them
       for (size_t m = 0; m < nm; ++m) {
@@ -236,12 +294,20 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
       RR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->generation = gen;
     }
+    std::vector<CopySeg> outs;
     for (size_t m = 0; m < nm; ++m) {
       const size_t cnt = b->t[m].n * b->t[m].n;
       if (!cnt) continue;
-      RR_HIP(hipMemcpyAsync(j_out[m], b->jout + b->o_off[m], cnt * sizeof(double), hipMemcpyDefault, c.stream));
-      RR_HIP(hipMemcpyAsync(k_out[m], b->kout + b->o_off[m], cnt * sizeof(double), hipMemcpyDefault, c.stream));
+      double* dst[2] = {j_out[m], k_out[m]};
+      const double* src[2] = {b->jout + b->o_off[m], b->kout + b->o_off[m]};
+      for (int q = 0; q < 2; ++q) {
+        if (rr::is_device_ptr(dst[q]))
+          outs.push_back({src[q], dst[q], cnt, 0});
+        else
+          RR_HIP(hipMemcpyAsync(dst[q], src[q], cnt * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      }
     }
+    segmented_copy(c, b, outs);
     if (replayed) *replayed = rep;
   });
 }

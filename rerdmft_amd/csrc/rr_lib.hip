@@ -441,11 +441,20 @@ __global__ void k_cplx_asm(CplxAsm a) {
   }
 }
 
-// Reduce split partials: out[e] = sum_s part[s*stride + e].
+// Reduce split partials: out[e] = sum_s part[s*stride + e], summed in split order. Eight loads are
+// issued before their adds so a thread has eight partials in flight.
 __global__ void k_reduce_parts(const double* part, int nsplit, size_t stride, size_t count, double* out) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (int z = 0; z < nsplit; ++z) s += part[z * stride + i];
+    int z = 0;
+    for (; z + 8 <= nsplit; z += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(z + u) * stride + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; z < nsplit; ++z) s += part[(size_t)z * stride + i];
     out[i] = s;
   }
 }
