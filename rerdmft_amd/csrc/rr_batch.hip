@@ -2,23 +2,16 @@
 // resident together, and one J/K build per molecule (factor form, D = L L^T) for the whole batch
 // in a single HIP-graph launch.
 //
-// The first call for a given set of factor ranks and precision runs the batch eagerly (this also
-// sizes the shared workspaces and lets the tile autotuner see every shape), then captures the same
-// sequence of launches into a graph; later calls copy the new factors into the batch's staging
-// buffer, replay the graph and copy the results out. Molecules run on up to eight parallel graph
-// branches, each with a private stream and workspace set, so small molecules overlap. A graph is
-// rebuilt whenever a workspace buffer it recorded has been reallocated since (a branch's workspace
-// generation changed).
+// Every phase of the per-molecule build (factor packing, half-transform, gamma, rank-k update,
+// Coulomb stream, reductions, assembly) runs as one grouped launch over all molecules (rr_lib.hip:
+// batch_plan_*), each molecule with the grid its single-molecule launch would use, so results equal
+// rr_ri_jk_factor bitwise. The first call for a given set of factor ranks and precision builds the
+// plan (segment tables and work buffers), runs it eagerly and captures its launches into a graph;
+// later calls copy the new factors into the batch's staging buffer, replay the graph and copy the
+// results out, each copy direction as one segmented copy launch.
 #include <algorithm>
 
 #include "rr_core.hpp"
-
-// One branch of the batch graph: a private stream and workspace set (kernel launches of different
-// branches share nothing but the read-only tensors and disjoint output regions).
-struct BatchLane {
-  rr::Context ctx;
-  hipEvent_t done = nullptr;
-};
 
 // One contiguous device-to-device copy of a segmented copy launch.
 struct CopySeg {
@@ -30,7 +23,6 @@ struct CopySeg {
 
 struct rr_batch {
   std::vector<rr::Tensor> t;
-  std::vector<BatchLane*> lanes;
   std::vector<size_t> l_off, o_off;  // offsets (doubles) into the staging buffers
   std::vector<size_t> r;             // factor ranks of the captured graph
   int precision = -1;
@@ -38,10 +30,9 @@ struct rr_batch {
   double* jout = nullptr;            // concatenated J, K (device)
   double* kout = nullptr;
   size_t lin_cap = 0, out_cap = 0;
+  rr::BatchPlan* plan = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<unsigned long long> generation;  // lane workspace generations at capture
-  hipEvent_t fork = nullptr;
   CopySeg* segs = nullptr;           // device table of the per-call gather / scatter copies
   size_t seg_cap = 0;
 };
@@ -87,8 +78,8 @@ bool segmented_copy(Context& c, rr_batch* b, const std::vector<CopySeg>& in) {
     RR_HIP(hipMalloc(&b->segs, segs.size() * sizeof(CopySeg)));
     b->seg_cap = segs.size();
   }
-  // the table is read by the kernel launched right after on the same stream; the synchronous copy
-  // from pageable memory returns once the table is in device memory
+  // the table is read by the kernel launched right after on the same stream; the host vector is
+  // released on return, so wait for the copy
   RR_HIP(hipMemcpyAsync(b->segs, segs.data(), segs.size() * sizeof(CopySeg), hipMemcpyHostToDevice, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));
   hipLaunchKernelGGL(k_segment_copy, dim3((unsigned)blocks), dim3(256), 0, c.stream, b->segs, (int)segs.size());
@@ -101,56 +92,18 @@ void drop_graph(rr_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  rr::batch_plan_free(b->plan);
+  b->plan = nullptr;
 }
 
-constexpr size_t kMaxLanes = 8;
-
-void free_lane(BatchLane* l) {
-  if (!l) return;
-  (void)hipStreamSynchronize(l->ctx.stream);
-  for (auto& kv : l->ctx.ws) (void)hipFree(kv.second.first);
-  if (l->ctx.zpage) (void)hipFree(l->ctx.zpage);
-  if (l->done) (void)hipEventDestroy(l->done);
-  if (l->ctx.stream) (void)hipStreamDestroy(l->ctx.stream);
-  delete l;
-}
-
-BatchLane* make_lane(int device) {
-  BatchLane* l = new BatchLane();
-  l->ctx.device = device;
-  if (hipStreamCreateWithFlags(&l->ctx.stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&l->done, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc(&l->ctx.zpage, 256) != hipSuccess || hipMemset(l->ctx.zpage, 0, 256) != hipSuccess) {
-    free_lane(l);
-    throw Error{RR_ERR_HIP, "batch: cannot create a graph branch"};
-  }
-  return l;
-}
-
-// Molecules are dealt round-robin to the lanes, largest first, each lane in order on its stream.
-// The lanes fork from and join back into the main stream through events (captured as graph edges).
-void enqueue_all(Context& c, rr_batch* b) {
-  RR_HIP(hipEventRecord(b->fork, c.stream));
-  std::vector<size_t> order(b->t.size());
-  for (size_t m = 0; m < order.size(); ++m) order[m] = m;
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return b->t[x].n > b->t[y].n; });
-  for (BatchLane* l : b->lanes) RR_HIP(hipStreamWaitEvent(l->ctx.stream, b->fork, 0));
-  for (size_t i = 0; i < order.size(); ++i) {
-    const size_t m = order[i];
-    Context& lc = b->lanes[i % b->lanes.size()]->ctx;
-    rr::enqueue_jk_factor(lc, b->t[m], b->precision, b->lin + b->l_off[m], b->r[m], b->jout + b->o_off[m],
-                          b->kout + b->o_off[m]);
-  }
-  for (BatchLane* l : b->lanes) {
-    RR_HIP(hipEventRecord(l->done, l->ctx.stream));
-    RR_HIP(hipStreamWaitEvent(c.stream, l->done, 0));
-  }
-}
-
-std::vector<unsigned long long> lane_generations(const rr_batch* b) {
-  std::vector<unsigned long long> g;
-  for (const BatchLane* l : b->lanes) g.push_back(l->ctx.ws_generation);
-  return g;
+void free_batch(rr_batch* b) {
+  drop_graph(b);
+  for (auto& t : b->t) rr::tensor_free(t);
+  if (b->lin) (void)hipFree(b->lin);
+  if (b->jout) (void)hipFree(b->jout);
+  if (b->kout) (void)hipFree(b->kout);
+  if (b->segs) (void)hipFree(b->segs);
+  delete b;
 }
 
 }  // namespace
@@ -177,17 +130,9 @@ int rr_batch_create(rr_handle h, size_t n_mol, const size_t* n_aux, const size_t
     b->out_cap = std::max<size_t>(off, 1);
     RR_HIP(hipMalloc(&b->jout, b->out_cap * sizeof(double)));
     RR_HIP(hipMalloc(&b->kout, b->out_cap * sizeof(double)));
-    RR_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-    const size_t nl = std::max<size_t>(1, std::min(kMaxLanes, n_mol));
-    for (size_t i = 0; i < nl; ++i) b->lanes.push_back(make_lane(c.device));
   });
   if (rc != RR_OK) {
-    for (BatchLane* l : b->lanes) free_lane(l);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    for (auto& t : b->t) rr::tensor_free(t);
-    if (b->jout) (void)hipFree(b->jout);
-    if (b->kout) (void)hipFree(b->kout);
-    delete b;
+    free_batch(b);
     return rc;
   }
   *out = b;
@@ -198,15 +143,7 @@ int rr_batch_destroy(rr_handle h, rr_batch_t b) {
   if (!h || !b) return RR_ERR_ARG;
   return guarded(h, [&](Context& c) {
     RR_HIP(hipStreamSynchronize(c.stream));
-    drop_graph(b);
-    for (BatchLane* l : b->lanes) free_lane(l);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    for (auto& t : b->t) rr::tensor_free(t);
-    if (b->lin) (void)hipFree(b->lin);
-    if (b->jout) (void)hipFree(b->jout);
-    if (b->kout) (void)hipFree(b->kout);
-    if (b->segs) (void)hipFree(b->segs);
-    delete b;
+    free_batch(b);
   });
 }
 
@@ -230,12 +167,11 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
       loff[m] = ltot;
       ltot += b->t[m].n * r[m];
     }
-    const bool same_shape = b->exec && b->precision == precision && b->r == std::vector<size_t>(r, r + nm) &&
-                            b->generation == lane_generations(b);
+    const bool same_shape = b->exec && b->precision == precision && b->r == std::vector<size_t>(r, r + nm);
     if (!same_shape) {
+      RR_HIP(hipStreamSynchronize(c.stream));
       drop_graph(b);
       if (ltot > b->lin_cap) {
-        RR_HIP(hipStreamSynchronize(c.stream));
         if (b->lin) RR_HIP(hipFree(b->lin));
         b->lin = nullptr;
         RR_HIP(hipMalloc(&b->lin, std::max<size_t>(ltot, 1) * sizeof(double)));
@@ -274,25 +210,20 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
       RR_HIP(hipGraphLaunch(b->exec, c.stream));
       rep = 1;
     } else {
-      enqueue_all(c, b);  // eager run: results of this call, workspace sizing, autotuning
+      b->plan = rr::batch_plan_create(c, b->t, precision, b->r, b->lin, b->l_off, b->jout, b->kout, b->o_off);
+      rr::batch_plan_enqueue(c, b->plan);  // eager run: the results of this call
       RR_HIP(hipStreamSynchronize(c.stream));
-      const std::vector<unsigned long long> gen = lane_generations(b);
-      const bool timing = rr::set_phase_timing(false);  // event timing synchronizes: not capturable
       RR_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
       try {
-        enqueue_all(c, b);
+        rr::batch_plan_enqueue(c, b->plan);
       } catch (...) {
         hipGraph_t dead = nullptr;
         (void)hipStreamEndCapture(c.stream, &dead);
         if (dead) (void)hipGraphDestroy(dead);
-        rr::set_phase_timing(timing);
         throw;
       }
-      rr::set_phase_timing(timing);
       RR_HIP(hipStreamEndCapture(c.stream, &b->graph));
-      RR_REQUIRE(lane_generations(b) == gen, RR_ERR_INTERNAL, "batch: workspace reallocated during graph capture");
       RR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-      b->generation = gen;
     }
     std::vector<CopySeg> outs;
     for (size_t m = 0; m < nm; ++m) {

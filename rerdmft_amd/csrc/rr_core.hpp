@@ -176,6 +176,15 @@ void mo_tensor_rkb(Context& c, int slot_ll, int slot_ss, size_t nl, size_t ns, c
                    double* dout);
 void mo_tensor_nonrel_embedded(Context& c, int slot, size_t n, const double* dC_spatial, double* dout);
 size_t slot_naux(Context& c, int slot);
+// A whole batch of factor-form J/K builds (one per tensor, D = L L^T) as a few grouped launches on
+// the context stream, with results identical to enqueue_jk_factor per tensor. L_m is read from
+// lin + loff[m] (n x r row-major), J_m and K_m are written to jout / kout + ooff[m].
+struct BatchPlan;
+BatchPlan* batch_plan_create(Context& c, const std::vector<Tensor>& t, int prec, const std::vector<size_t>& r,
+                             const double* lin, const std::vector<size_t>& loff, double* jout, double* kout,
+                             const std::vector<size_t>& ooff);
+void batch_plan_enqueue(Context& c, const BatchPlan* p);
+void batch_plan_free(BatchPlan* p);
 // Tensor storage outside the numbered slots (batches of molecules).
 void tensor_free(Tensor& t);
 void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t n, unsigned copies);

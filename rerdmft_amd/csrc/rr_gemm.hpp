@@ -115,8 +115,9 @@ __device__ inline void cplx_split(const Mma<bf16_t>::frag& b, Mma<bf16_t>::frag&
   fi.v = __builtin_bit_cast(bf8, i);
 }
 
+// One workgroup's tile of gemm_nt_kernel; w = linear workgroup index (tile, batch, split).
 template <typename TI, typename TO, int WM, int WN, bool CPLX>
-__global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_nt_tile(const GemmArgs& g, int w) {
   using MM = Mma<TI>;
   using acc_t = typename MM::acc_t;
   using frag = typename MM::frag;
@@ -125,9 +126,6 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
   constexpr int CW = CPLX ? 2 : 1;
   constexpr int BM = 32 * WM, BN = 32 * WN;
 
-  const int total = g.ntiles * g.nbatch * g.nsplit;
-  int w = static_cast<int>(blockIdx.x);
-  if (g.xcd_remap) w = remap_xcd(w, total);
   const int tile = w % g.ntiles;
   const int rest = w / g.ntiles;
   const int batch = rest % g.nbatch;
@@ -270,6 +268,14 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
     }
 }
 
+template <typename TI, typename TO, int WM, int WN, bool CPLX>
+__global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
+  const int total = g.ntiles * g.nbatch * g.nsplit;
+  int w = static_cast<int>(blockIdx.x);
+  if (g.xcd_remap) w = remap_xcd(w, total);
+  gemm_nt_tile<TI, TO, WM, WN, CPLX>(g, w);
+}
+
 }  // namespace rr
 
 namespace rr {
@@ -283,7 +289,7 @@ namespace rr {
 // per stage. Supports batches, split-K partials, lower-triangle tiles and the second output.
 // ------------------------------------------------------------------------------------------------
 template <typename TI, typename TO, int BM, int BN, int BK, int WGM, int NBUF = 2>
-__global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_nt_lds_tile(const GemmArgs& g, int w) {
   static_assert(NBUF == 1 || NBUF == 2, "one or two LDS stage buffers");
   using MM = Mma<TI>;
   using acc_t = typename MM::acc_t;
@@ -304,9 +310,6 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
 
   __shared__ __attribute__((aligned(16))) TI lds[NBUF][(BM + BN) * LDR];
 
-  const int total = g.ntiles * g.nbatch * g.nsplit;
-  int w = static_cast<int>(blockIdx.x);
-  if (g.xcd_remap) w = remap_xcd(w, total);
   const int tile = w % g.ntiles;
   const int rest = w / g.ntiles;
   const int batch = rest % g.nbatch;
@@ -453,6 +456,14 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
         store_out<TO>(p, v);
       }
     }
+}
+
+template <typename TI, typename TO, int BM, int BN, int BK, int WGM, int NBUF = 2>
+__global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g) {
+  const int total = g.ntiles * g.nbatch * g.nsplit;
+  int w = static_cast<int>(blockIdx.x);
+  if (g.xcd_remap) w = remap_xcd(w, total);
+  gemm_nt_lds_tile<TI, TO, BM, BN, BK, WGM, NBUF>(g, w);
 }
 
 

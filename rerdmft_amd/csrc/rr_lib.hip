@@ -122,15 +122,20 @@ __global__ void k_convert(const TI* in, TO* out, size_t count) {
 // out[r][c] (ld_out) = scale * op(in)[r][c] for r < Rv, c < Cc (op = transpose when trans); rows
 // [Rv, R) and columns [Cc, ld_out) are zero-filled.
 template <typename TO>
-__global__ void k_pack(const double* in, size_t ld_in, int trans, size_t R, size_t Rv, size_t Cc, size_t ld_out,
-                       double scale, TO* out) {
+__device__ __forceinline__ void k_pack_body(const double* in, size_t ld_in, int trans, size_t R, size_t Rv, size_t Cc, size_t ld_out,
+                       double scale, TO* out, unsigned bx_, unsigned by_, unsigned gx_) {
   const size_t total = R * ld_out;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+  for (size_t idx = bx_ * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gx_ * blockDim.x) {
     const size_t r = idx / ld_out, c = idx % ld_out;
     double v = 0.0;
     if (c < Cc && r < Rv) v = scale * (trans ? in[c * ld_in + r] : in[r * ld_in + c]);
     out[idx] = from_double<TO>(v);
   }
+}
+template <typename TO>
+__global__ void k_pack(const double* in, size_t ld_in, int trans, size_t R, size_t Rv, size_t Cc, size_t ld_out,
+                       double scale, TO* out) {
+  k_pack_body<TO>(in, ld_in, trans, R, Rv, Cc, ld_out, scale, out, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 __global__ void k_conj_inplace(double* a, size_t count) {
@@ -240,12 +245,12 @@ __global__ void k_gamma_reduce(const double* gpart, int nsplit, size_t naux, dou
 
 // jpart[split][e] = sum_{P in split range} gamma[P] * Bt[P][e]; each thread owns U 16-byte vectors.
 template <typename TB, int U>
-__global__ __launch_bounds__(256) void k_coulomb(const TB* B, size_t rowlen, size_t naux, const double* gamma,
-                                                 size_t p_per_split, double* jpart) {
+__device__ __forceinline__ void k_coulomb_body(const TB* B, size_t rowlen, size_t naux, const double* gamma,
+                                                 size_t p_per_split, double* jpart, unsigned bx_, unsigned by_, unsigned gx_) {
   using V = Vec16<TB>;
   constexpr int VN = V::N;
-  const size_t base = (size_t)blockIdx.x * (256 * VN * U) + (size_t)threadIdx.x * VN;
-  const size_t split = blockIdx.y;
+  const size_t base = (size_t)bx_ * (256 * VN * U) + (size_t)threadIdx.x * VN;
+  const size_t split = by_;
   const size_t pb = split * p_per_split, pe = std::min(naux, pb + p_per_split);
   double acc[U][VN];
 #pragma unroll
@@ -276,6 +281,11 @@ __global__ __launch_bounds__(256) void k_coulomb(const TB* B, size_t rowlen, siz
       for (int q = 0; q < VN; ++q) out[base + (size_t)u * 256 * VN + q] = acc[u][q];
     }
 }
+template <typename TB, int U>
+__global__ __launch_bounds__(256) void k_coulomb(const TB* B, size_t rowlen, size_t naux, const double* gamma,
+                                                 size_t p_per_split, double* jpart) {
+  k_coulomb_body<TB, U>(B, rowlen, naux, gamma, p_per_split, jpart, blockIdx.x, blockIdx.y, gridDim.x);
+}
 
 // Upper triangle of a symmetric tensor row in 16-byte vectors: row nu holds the vectors from the one
 // containing the diagonal (index floor(nu / VN)) to the end of the padded row. Rows come in groups of
@@ -302,13 +312,13 @@ inline size_t tri_vectors(size_t n, size_t ld, size_t VN) {
 // k_coulomb over the upper triangle only (symmetric tensors): jpart[split][e] for the triangle
 // elements e; the rest of jpart is left unwritten.
 template <typename TB, int U>
-__global__ __launch_bounds__(256) void k_coulomb_tri(const TB* B, size_t rowlen, size_t ld, size_t nvec, size_t naux,
-                                                     const double* gamma, size_t p_per_split, double* jpart) {
+__device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, size_t ld, size_t nvec, size_t naux,
+                                                     const double* gamma, size_t p_per_split, double* jpart, unsigned bx_, unsigned by_, unsigned gx_) {
   using V = Vec16<TB>;
   constexpr int VN = V::N;
   const size_t W = ld / VN;
-  const size_t vbase = (size_t)blockIdx.x * (256 * U) + threadIdx.x;
-  const size_t split = blockIdx.y;
+  const size_t vbase = (size_t)bx_ * (256 * U) + threadIdx.x;
+  const size_t split = by_;
   const size_t pb = split * p_per_split, pe = std::min(naux, pb + p_per_split);
   size_t e[U];
   bool valid[U];
@@ -344,6 +354,11 @@ __global__ __launch_bounds__(256) void k_coulomb_tri(const TB* B, size_t rowlen,
       for (int q = 0; q < VN; ++q) out[e[u] + q] = acc[u][q];
     }
 }
+template <typename TB, int U>
+__global__ __launch_bounds__(256) void k_coulomb_tri(const TB* B, size_t rowlen, size_t ld, size_t nvec, size_t naux,
+                                                     const double* gamma, size_t p_per_split, double* jpart) {
+  k_coulomb_tri_body<TB, U>(B, rowlen, ld, nvec, naux, gamma, p_per_split, jpart, blockIdx.x, blockIdx.y, gridDim.x);
+}
 
 // ================================================================================================
 // Epilogues
@@ -368,9 +383,9 @@ struct RealAsm {
   double* K = nullptr;
 };
 
-__global__ void k_real_asm(RealAsm a) {
+__device__ __forceinline__ void k_real_asm_body(RealAsm a, unsigned bx_, unsigned by_, unsigned gx_) {
   const size_t total = a.n * a.n;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+  for (size_t idx = bx_ * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gx_ * blockDim.x) {
     const size_t mu = idx / a.n, nu = idx % a.n;
     double j = 0.0;
     if (a.jpart) {
@@ -394,6 +409,9 @@ __global__ void k_real_asm(RealAsm a) {
     if (a.K) a.K[idx] = k;
     if (a.F) a.F[idx] = (a.h ? a.h[idx] : 0.0) + j + a.kfac * k;
   }
+}
+__global__ void k_real_asm(RealAsm a) {
+  k_real_asm_body(a, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // Complex Hermitian assembly over nb diagonal Coulomb blocks: out[a][b] (complex, dim nd) =
@@ -443,8 +461,8 @@ __global__ void k_cplx_asm(CplxAsm a) {
 
 // Reduce split partials: out[e] = sum_s part[s*stride + e], summed in split order. Eight loads are
 // issued before their adds so a thread has eight partials in flight.
-__global__ void k_reduce_parts(const double* part, int nsplit, size_t stride, size_t count, double* out) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void k_reduce_parts_body(const double* part, int nsplit, size_t stride, size_t count, double* out, unsigned bx_, unsigned by_, unsigned gx_) {
+  for (size_t i = bx_ * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gx_ * blockDim.x) {
     double s = 0.0;
     int z = 0;
     for (; z + 8 <= nsplit; z += 8) {
@@ -457,6 +475,9 @@ __global__ void k_reduce_parts(const double* part, int nsplit, size_t stride, si
     for (; z < nsplit; ++z) s += part[(size_t)z * stride + i];
     out[i] = s;
   }
+}
+__global__ void k_reduce_parts(const double* part, int nsplit, size_t stride, size_t count, double* out) {
+  k_reduce_parts_body(part, nsplit, stride, count, out, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // ================================================================================================
@@ -1005,13 +1026,13 @@ struct ExchangeOut {
 // Pass 1: each thread owns 8 consecutive flat k = (P,i) positions (one aux row P, rp % 8 == 0) and
 // sums over a slice of nu rows; pass 2 reduces the slices and the rp/8 groups of each P in order.
 template <typename TW>
-__global__ __launch_bounds__(256) void k_gamma_w_partial(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
-                                                         const double* dfac, size_t lddf, size_t rows, double* part) {
-  const size_t g8 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;  // group of 8 k
+__device__ __forceinline__ void k_gamma_w_partial_body(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+                                                         const double* dfac, size_t lddf, size_t rows, double* part, unsigned bx_, unsigned by_, unsigned gx_) {
+  const size_t g8 = bx_ * (size_t)blockDim.x + threadIdx.x;  // group of 8 k
   const size_t k0 = g8 * 8;
   if (k0 >= kdim) return;
   const size_t i0 = k0 % rp;
-  const size_t nu0 = blockIdx.y * rows, nu1 = std::min(n, nu0 + rows);
+  const size_t nu0 = by_ * rows, nu1 = std::min(n, nu0 + rows);
   double acc = 0.0;
   for (size_t nu = nu0; nu < nu1; ++nu) {
     const TW* w = W + nu * kdim + k0;
@@ -1024,7 +1045,12 @@ __global__ __launch_bounds__(256) void k_gamma_w_partial(const TW* W, const TW* 
       for (int q = 0; q < 8; ++q) acc += Conv<TW>::from(w2[q]) * df[rp + q];
     }
   }
-  part[blockIdx.y * (kdim / 8) + g8] = acc;
+  part[by_ * (kdim / 8) + g8] = acc;
+}
+template <typename TW>
+__global__ __launch_bounds__(256) void k_gamma_w_partial(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+                                                         const double* dfac, size_t lddf, size_t rows, double* part) {
+  k_gamma_w_partial_body<TW>(W, W2, kdim, rp, n, dfac, lddf, rows, part, blockIdx.x, blockIdx.y, gridDim.x);
 }
 // Eight consecutive elements (16-byte aligned) widened to double.
 template <typename TW>
@@ -1038,14 +1064,14 @@ __device__ inline void load8(const TW* p, double* o) {
 // with each thread owning one group of 8 columns i for NPP consecutive aux rows P: the 8 weights of a
 // row nu are loaded once for NPP rows of W, and every load is a 16-byte vector.
 template <typename TW, int NPP>
-__global__ __launch_bounds__(256) void k_gamma_w_blocks(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+__device__ __forceinline__ void k_gamma_w_blocks_body(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
                                                         size_t naux, const double* dfac, size_t lddf, size_t rows,
-                                                        double* part) {
+                                                        double* part, unsigned bx_, unsigned by_, unsigned gx_) {
   const size_t g8n = rp / 8, nblk = (naux + NPP - 1) / NPP;
-  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t t = bx_ * (size_t)blockDim.x + threadIdx.x;
   if (t >= nblk * g8n) return;
   const size_t q = t % g8n, i0 = 8 * q, P0 = (t / g8n) * NPP;
-  const size_t nu0 = blockIdx.y * rows, nu1 = std::min(n, nu0 + rows);
+  const size_t nu0 = by_ * rows, nu1 = std::min(n, nu0 + rows);
   double acc[NPP];
 #pragma unroll
   for (int p = 0; p < NPP; ++p) acc[p] = 0.0;
@@ -1076,18 +1102,27 @@ __global__ __launch_bounds__(256) void k_gamma_w_blocks(const TW* W, const TW* W
   }
 #pragma unroll
   for (int p = 0; p < NPP; ++p)
-    if (P0 + p < naux) part[blockIdx.y * (kdim / 8) + (P0 + p) * g8n + q] = acc[p];
+    if (P0 + p < naux) part[by_ * (kdim / 8) + (P0 + p) * g8n + q] = acc[p];
+}
+template <typename TW, int NPP>
+__global__ __launch_bounds__(256) void k_gamma_w_blocks(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+                                                        size_t naux, const double* dfac, size_t lddf, size_t rows,
+                                                        double* part) {
+  k_gamma_w_blocks_body<TW, NPP>(W, W2, kdim, rp, n, naux, dfac, lddf, rows, part, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 
-__global__ void k_gamma_w_reduce(const double* part, int nsplit, size_t kdim, size_t rp, size_t naux, double* gamma) {
+__device__ __forceinline__ void k_gamma_w_reduce_body(const double* part, int nsplit, size_t kdim, size_t rp, size_t naux, double* gamma, unsigned bx_, unsigned by_, unsigned gx_) {
   const size_t gpp = rp / 8, ng = kdim / 8;
-  for (size_t P = blockIdx.x * (size_t)blockDim.x + threadIdx.x; P < naux; P += (size_t)gridDim.x * blockDim.x) {
+  for (size_t P = bx_ * (size_t)blockDim.x + threadIdx.x; P < naux; P += (size_t)gx_ * blockDim.x) {
     double s = 0.0;
     for (int z = 0; z < nsplit; ++z)
       for (size_t q = 0; q < gpp; ++q) s += part[z * ng + P * gpp + q];
     gamma[P] = s;
   }
+}
+__global__ void k_gamma_w_reduce(const double* part, int nsplit, size_t kdim, size_t rp, size_t naux, double* gamma) {
+  k_gamma_w_reduce_body(part, nsplit, kdim, rp, naux, gamma, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // gamma[P] = sum_(nu,i) W[nu][(P,i)] dfac[nu][i] (+ W2 with dfac columns [rp, 2rp)) on the stream.
@@ -1121,16 +1156,20 @@ void gamma_from_w(Context& c, const TW* W, const TW* W2, size_t kdim, size_t rp,
 
 // dfac (n x ncols) for the Coulomb contraction of a factor-form density: column block b of the
 // half-transformed W is weighted by rows of fac[b] (r x ld_l, i.e. factor^T), scaled by scale[b].
-__global__ void k_build_dfac(const double* f0, const double* f1, size_t r, size_t ld_l, size_t n, size_t rp, int nblk,
-                             double s0, double s1, double* out) {
+__device__ __forceinline__ void k_build_dfac_body(const double* f0, const double* f1, size_t r, size_t ld_l, size_t n, size_t rp, int nblk,
+                             double s0, double s1, double* out, unsigned bx_, unsigned by_, unsigned gx_) {
   const size_t cols = nblk * rp;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n * cols; idx += (size_t)gridDim.x * blockDim.x) {
+  for (size_t idx = bx_ * (size_t)blockDim.x + threadIdx.x; idx < n * cols; idx += (size_t)gx_ * blockDim.x) {
     const size_t nu = idx / cols, col = idx % cols, b = col / rp, i = col % rp;
     double v = 0.0;
     const double* f = b == 0 ? f0 : f1;
     if (i < r && f) v = (b == 0 ? s0 : s1) * f[i * ld_l + nu];
     out[idx] = v;
   }
+}
+__global__ void k_build_dfac(const double* f0, const double* f1, size_t r, size_t ld_l, size_t n, size_t rp, int nblk,
+                             double s0, double s1, double* out) {
+  k_build_dfac_body(f0, f1, r, ld_l, n, rp, nblk, s0, s1, out, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 template <typename TI>
@@ -2538,6 +2577,419 @@ void enqueue_jk_factor(Context& c, const Tensor& t, int prec, const double* dL, 
   double* jpart = nullptr;
   const int jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart, t.symmetric);
   real_assemble(c, t, jpart, jsplit, &ex, -0.5, nullptr, nullptr, dJ, dK, t.symmetric);
+}
+
+// ================================================================================================
+// Grouped launches: one launch runs a kernel body over many independent problems (segments), each
+// with its own arguments and block count; block b runs segment s (first[s] <= b < first[s+1]) as its
+// block b - first[s]. Used to build a whole batch of molecules in a few launches. Every segment
+// gets the grid the single-problem launch would use, so results are identical to those launches.
+// ================================================================================================
+template <typename Op>
+__global__ __launch_bounds__(256) void k_grouped(const typename Op::Args* args, const unsigned* first, int nseg) {
+  const unsigned b = blockIdx.x;
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const typename Op::Args a = args[lo];  // uniform: held in scalar registers
+  Op::run(a, b - first[lo]);
+}
+
+template <typename TO>
+struct OpPack {
+  struct Args {
+    const double* in;
+    size_t ld_in;
+    int trans;
+    size_t R, Rv, Cc, ld_out;
+    double scale;
+    TO* out;
+    unsigned gx;
+  };
+  __device__ static void run(const Args& a, unsigned lb) {
+    k_pack_body<TO>(a.in, a.ld_in, a.trans, a.R, a.Rv, a.Cc, a.ld_out, a.scale, a.out, lb, 0, a.gx);
+  }
+};
+struct OpDfac {
+  struct Args {
+    const double* f0;
+    size_t r, ld_l, n, rp;
+    double* out;
+    unsigned gx;
+  };
+  __device__ static void run(const Args& a, unsigned lb) {
+    k_build_dfac_body(a.f0, nullptr, a.r, a.ld_l, a.n, a.rp, 1, 1.0, 0.0, a.out, lb, 0, a.gx);
+  }
+};
+template <typename TW>
+struct OpGammaW {  // k_gamma_w_partial (fp64) or k_gamma_w_blocks<4> (fp32 / bf16)
+  struct Args {
+    const TW* W;
+    size_t kdim, rp, n, naux;
+    const double* dfac;
+    size_t lddf, rows;
+    double* part;
+    unsigned gx;
+  };
+  __device__ static void run(const Args& a, unsigned lb) {
+    if constexpr (std::is_same<TW, double>::value)
+      k_gamma_w_partial_body<TW>(a.W, nullptr, a.kdim, a.rp, a.n, a.dfac, a.lddf, a.rows, a.part, lb % a.gx, lb / a.gx, a.gx);
+    else
+      k_gamma_w_blocks_body<TW, 4>(a.W, nullptr, a.kdim, a.rp, a.n, a.naux, a.dfac, a.lddf, a.rows, a.part, lb % a.gx,
+                                   lb / a.gx, a.gx);
+  }
+};
+struct OpGammaReduce {
+  struct Args {
+    const double* part;
+    int nsplit;
+    size_t kdim, rp, naux;
+    double* gamma;
+    unsigned gx;
+  };
+  __device__ static void run(const Args& a, unsigned lb) {
+    k_gamma_w_reduce_body(a.part, a.nsplit, a.kdim, a.rp, a.naux, a.gamma, lb, 0, a.gx);
+  }
+};
+template <typename TB>
+struct OpCoulomb {  // k_coulomb_tri (symmetric tensors) or k_coulomb, U = 4
+  struct Args {
+    const TB* B;
+    size_t rowlen, ld, nvec, naux;
+    const double* gamma;
+    size_t pps;
+    double* jpart;
+    int tri;
+    unsigned gx;
+  };
+  __device__ static void run(const Args& a, unsigned lb) {
+    if (a.tri)
+      k_coulomb_tri_body<TB, 4>(a.B, a.rowlen, a.ld, a.nvec, a.naux, a.gamma, a.pps, a.jpart, lb % a.gx, lb / a.gx, a.gx);
+    else
+      k_coulomb_body<TB, 4>(a.B, a.rowlen, a.naux, a.gamma, a.pps, a.jpart, lb % a.gx, lb / a.gx, a.gx);
+  }
+};
+struct OpReduce {
+  struct Args {
+    const double* part;
+    int nsplit;
+    size_t stride, count;
+    double* out;
+    unsigned gx;
+  };
+  __device__ static void run(const Args& a, unsigned lb) {
+    k_reduce_parts_body(a.part, a.nsplit, a.stride, a.count, a.out, lb, 0, a.gx);
+  }
+};
+struct OpAsm {
+  struct Args {
+    RealAsm a;
+    unsigned gx;
+  };
+  __device__ static void run(const Args& a, unsigned lb) { k_real_asm_body(a.a, lb, 0, a.gx); }
+};
+template <typename TI, typename TO, int BM, int BN, int BK, int WGM>
+struct OpGemmLds {
+  using Args = GemmArgs;
+  __device__ static void run(const Args& a, unsigned lb) { gemm_nt_lds_tile<TI, TO, BM, BN, BK, WGM, 2>(a, (int)lb); }
+};
+
+// Host side of one grouped launch: the segment table, then one contiguous device copy of it.
+template <typename Op>
+struct Grouped {
+  std::vector<typename Op::Args> args;
+  std::vector<unsigned> first{0};
+  const typename Op::Args* dargs = nullptr;
+  const unsigned* dfirst = nullptr;
+  void add(const typename Op::Args& a, size_t blocks) {
+    if (!blocks) return;
+    RR_REQUIRE(first.back() + blocks < (1ull << 31), RR_ERR_ARG, "batch: grouped launch too large");
+    args.push_back(a);
+    first.push_back(first.back() + (unsigned)blocks);
+  }
+  size_t bytes() const { return roundup(args.size() * sizeof(typename Op::Args), 256) + roundup(first.size() * 4, 256); }
+  void upload(Context& c, char*& at) {
+    if (args.empty()) return;
+    const size_t ab = args.size() * sizeof(typename Op::Args);
+    RR_HIP(hipMemcpyAsync(at, args.data(), ab, hipMemcpyHostToDevice, c.stream));
+    dargs = reinterpret_cast<const typename Op::Args*>(at);
+    at += roundup(ab, 256);
+    RR_HIP(hipMemcpyAsync(at, first.data(), first.size() * 4, hipMemcpyHostToDevice, c.stream));
+    dfirst = reinterpret_cast<const unsigned*>(at);
+    at += roundup(first.size() * 4, 256);
+  }
+  void launch(hipStream_t s) const {
+    if (args.empty()) return;
+    hipLaunchKernelGGL(k_grouped<Op>, dim3(first.back()), dim3(256), 0, s, dargs, dfirst, (int)args.size());
+  }
+};
+
+// A batch of factor-form J/K builds (enqueue_jk_factor per molecule) as grouped launches; TI is the
+// operand type. The per-molecule work buffers are carved from one allocation.
+template <typename TI>
+struct BatchPlanT {
+  Grouped<OpPack<double>> pack;
+  Grouped<OpDfac> dfac;
+  Grouped<OpPack<TI>> pack_t;
+  // half-transform: 64 x 48 tiles (N = padded factor rank, at most a few tens here)
+  Grouped<OpGemmLds<TI, TI, 64, 48, 4 * Mma<TI>::KL * (std::is_same<TI, bf16_t>::value ? 2 : 1), 4>> gemm1;
+  Grouped<OpGammaW<TI>> gam;
+  Grouped<OpGammaReduce> gam_red;
+  // rank-k update: 64 x 64 lower tiles
+  Grouped<OpGemmLds<TI, double, 64, 64, 4 * Mma<TI>::KL * (std::is_same<TI, bf16_t>::value ? 2 : 1), 2>> gemm2;
+  Grouped<OpCoulomb<TI>> coul;
+  Grouped<OpReduce> red;
+  Grouped<OpAsm> asmb;
+  void* work = nullptr;
+  void* tables = nullptr;
+  // The Coulomb stream (reads the tensors) runs on the auxiliary stream beside the rank-k update
+  // (reads W) when the context has one.
+  void launch(Context& c) const {
+    const hipStream_t s = c.stream;
+    pack.launch(s);
+    dfac.launch(s);
+    pack_t.launch(s);
+    gemm1.launch(s);
+    gam.launch(s);
+    gam_red.launch(s);
+    if (c.aux) {
+      RR_HIP(hipEventRecord(c.ev_fork, s));
+      RR_HIP(hipStreamWaitEvent(c.aux, c.ev_fork, 0));
+      coul.launch(c.aux);
+      RR_HIP(hipEventRecord(c.ev_join, c.aux));
+      gemm2.launch(s);
+      RR_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
+    } else {
+      gemm2.launch(s);
+      coul.launch(s);
+    }
+    red.launch(s);
+    asmb.launch(s);
+  }
+};
+
+struct BatchPlan {
+  int prec = P64;
+  BatchPlanT<double>* p64 = nullptr;
+  BatchPlanT<float>* p32 = nullptr;
+  BatchPlanT<bf16_t>* p16 = nullptr;
+};
+
+namespace {
+template <typename TI>
+BatchPlanT<TI>* build_plan(Context& c, const std::vector<Tensor>& ts, int prec, const std::vector<size_t>& rs,
+                           const double* lin, const std::vector<size_t>& loff, double* jout, double* kout,
+                           const std::vector<size_t>& ooff) {
+  auto* p = new BatchPlanT<TI>();
+  try {
+    // pass 1: work-buffer layout (bytes, 256-aligned regions)
+    struct Reg {
+      size_t lt, dfac, ltT, W, gpart, gamma, kpart, jpart, jr, kr;
+    };
+    std::vector<Reg> reg(ts.size());
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = at;
+      at += roundup(std::max<size_t>(bytes, 1), 256);
+      return o;
+    };
+    const int slab = prec == P64 ? 16 : 32;
+    for (size_t m = 0; m < ts.size(); ++m) {
+      const Tensor& t = ts[m];
+      const size_t n = t.n, ld = t.ld, naux = t.naux, r = rs[m];
+      if (r == 0 || naux == 0) continue;
+      const size_t rp = roundup(r, 8), kdim = naux * rp;
+      Reg& g = reg[m];
+      g.lt = take(r * n * sizeof(double));
+      g.dfac = take(n * rp * sizeof(double));
+      g.ltT = take(rp * ld * sizeof(TI));
+      g.W = take(n * kdim * sizeof(TI));
+      g.gpart = take(n * (kdim / 8) * sizeof(double));  // at most n row slices
+      g.gamma = take(naux * sizeof(double));
+      const int ns = gemm2_splits(n, kdim, slab);
+      const size_t kper = roundup((kdim + ns - 1) / ns, slab);
+      const size_t nsplit = (kdim + kper - 1) / kper;
+      g.kpart = take(nsplit * n * n * sizeof(double));
+      g.jpart = take(32 * n * ld * sizeof(double));
+      g.jr = take(n * ld * sizeof(double));
+      g.kr = take(n * n * sizeof(double));
+    }
+    RR_HIP(hipMalloc(&p->work, std::max<size_t>(at, 256)));
+    char* base = static_cast<char*>(p->work);
+    // pass 2: the segments, with the grids of the single-molecule launches
+    for (size_t m = 0; m < ts.size(); ++m) {
+      const Tensor& t = ts[m];
+      const size_t n = t.n, ld = t.ld, naux = t.naux, r = rs[m];
+      double* dJ = jout + ooff[m];
+      double* dK = kout + ooff[m];
+      if (r == 0 || naux == 0) {  // J = K = 0
+        RealAsm a;
+        a.n = n, a.ld = ld, a.J = dJ, a.K = dK;
+        const unsigned gx = grid_for(n * n);
+        p->asmb.add({a, gx}, gx);
+        continue;
+      }
+      const Reg& g = reg[m];
+      const size_t rp = roundup(r, 8), kdim = naux * rp;
+      double* lt = reinterpret_cast<double*>(base + g.lt);
+      double* dfac = reinterpret_cast<double*>(base + g.dfac);
+      TI* ltT = reinterpret_cast<TI*>(base + g.ltT);
+      TI* W = reinterpret_cast<TI*>(base + g.W);
+      double* gpart = reinterpret_cast<double*>(base + g.gpart);
+      double* gamma = reinterpret_cast<double*>(base + g.gamma);
+      double* kpart = reinterpret_cast<double*>(base + g.kpart);
+      double* jpart = reinterpret_cast<double*>(base + g.jpart);
+      double* jr = reinterpret_cast<double*>(base + g.jr);
+      double* kr = reinterpret_cast<double*>(base + g.kr);
+      {
+        const unsigned gx = grid_for(r * n);
+        p->pack.add({lin + loff[m], r, 1, r, r, n, n, 1.0, lt, gx}, gx);
+      }
+      {
+        const unsigned gx = grid_for(n * rp);
+        p->dfac.add({lt, r, n, n, rp, dfac, gx}, gx);
+      }
+      {
+        const unsigned gx = grid_for(rp * ld);
+        p->pack_t.add({lt, n, 0, rp, r, n, ld, 1.0, ltT, gx}, gx);
+      }
+      {  // W = Bt_P Lt^T
+        GemmArgs a;
+        a.A = tensor_data(t, prec), a.lda = ld, a.sAb = n * ld;
+        a.B = ltT, a.ldb = ld, a.sBb = 0;
+        a.C = W, a.ldcm = kdim, a.ldcn = 1, a.sCb = rp;
+        a.M = (int)n, a.N = (int)rp, a.K = (int)roundup(n, 8), a.nbatch = (int)naux;
+        a.tiles_m = (a.M + 63) / 64, a.tiles_n = (a.N + 47) / 48, a.ntiles = a.tiles_m * a.tiles_n;
+        a.kper = a.K, a.nsplit = 1, a.xcd_remap = 0, a.zpage = c.zpage;
+        p->gemm1.add(a, (size_t)a.ntiles * naux);
+      }
+      int gsplit;
+      {  // gamma from W (gamma_from_w)
+        size_t gx, rows;
+        if constexpr (std::is_same<TI, double>::value) {
+          gx = ((kdim / 8) + 255) / 256;
+        } else {
+          gx = ((((naux + 3) / 4) * (rp / 8)) + 255) / 256;
+        }
+        rows = std::max<size_t>(8, (n * gx + 4095) / 4096);
+        gsplit = (int)((n + rows - 1) / rows);
+        p->gam.add({W, kdim, rp, n, naux, dfac, rp, rows, gpart, (unsigned)gx}, gx * (size_t)gsplit);
+        const unsigned gr = grid_for(naux);
+        p->gam_red.add({gpart, gsplit, kdim, rp, naux, gamma, gr}, gr);
+      }
+      int ksplit;
+      {  // K partials = W W^T, lower tiles, split over k
+        const int ns = gemm2_splits(n, kdim, slab);
+        const size_t kper = roundup((kdim + ns - 1) / ns, slab);
+        ksplit = (int)((kdim + kper - 1) / kper);
+        GemmArgs a;
+        a.A = W, a.lda = kdim, a.B = W, a.ldb = kdim;
+        a.C = kpart, a.ldcm = n, a.ldcn = 1, a.sCz = n * n;
+        a.M = (int)n, a.N = (int)n, a.K = (int)kdim, a.kper = (int)kper, a.nsplit = ksplit, a.tri = 1;
+        a.tiles_m = (a.M + 63) / 64, a.tiles_n = a.tiles_m, a.ntiles = a.tiles_m * (a.tiles_m + 1) / 2;
+        a.xcd_remap = 0, a.zpage = c.zpage;
+        p->gemm2.add(a, (size_t)a.ntiles * ksplit);
+      }
+      int jsplit;
+      {  // J partials (coulomb_stream)
+        const bool tri = t.symmetric;
+        const size_t rowlen = n * ld, vn = 16 / sizeof(TI), U = 4;
+        const size_t nvec = tri ? tri_vectors(n, ld, vn) : 0;
+        const size_t nx = tri ? (nvec + 256 * U - 1) / (256 * U) : (rowlen + 256 * vn * U - 1) / (256 * vn * U);
+        size_t js = std::max<size_t>(1, std::min<size_t>(32, (2048 + nx - 1) / nx));
+        js = std::min(js, std::max<size_t>(1, naux));
+        const size_t pps = (naux + js - 1) / js;
+        js = (naux + pps - 1) / pps;
+        jsplit = (int)js;
+        const size_t gx = std::max<size_t>(nx, 1);
+        p->coul.add({static_cast<const TI*>(tensor_data(t, prec)), rowlen, ld, nvec, naux, gamma, pps, jpart, tri ? 1 : 0,
+                     (unsigned)gx},
+                    gx * js);
+      }
+      // real_assemble: reduce the split partials, then assemble J and K
+      RealAsm a;
+      a.n = n, a.ld = ld, a.jtri = t.symmetric ? 1 : 0;
+      a.jpart = jpart, a.jsplit = jsplit, a.jstride = n * ld;
+      if (jsplit > 1) {
+        const unsigned gx = grid_for(n * ld);
+        p->red.add({jpart, jsplit, n * ld, n * ld, jr, gx}, gx);
+        a.jpart = jr, a.jsplit = 1;
+      }
+      a.kpart = kpart, a.ksplit = ksplit, a.kstride = n * n, a.ktile = 1;
+      if (ksplit > 1) {
+        const unsigned gx = grid_for(n * n);
+        p->red.add({kpart, ksplit, n * n, n * n, kr, gx}, gx);
+        a.kpart = kr, a.ksplit = 1;
+      }
+      a.kfac = -0.5, a.J = dJ, a.K = dK;
+      const unsigned gx = grid_for(n * n);
+      p->asmb.add({a, gx}, gx);
+    }
+    const size_t tb = p->pack.bytes() + p->dfac.bytes() + p->pack_t.bytes() + p->gemm1.bytes() + p->gam.bytes() +
+                      p->gam_red.bytes() + p->gemm2.bytes() + p->coul.bytes() + p->red.bytes() + p->asmb.bytes();
+    RR_HIP(hipMalloc(&p->tables, std::max<size_t>(tb, 256)));
+    char* tat = static_cast<char*>(p->tables);
+    p->pack.upload(c, tat);
+    p->dfac.upload(c, tat);
+    p->pack_t.upload(c, tat);
+    p->gemm1.upload(c, tat);
+    p->gam.upload(c, tat);
+    p->gam_red.upload(c, tat);
+    p->gemm2.upload(c, tat);
+    p->coul.upload(c, tat);
+    p->red.upload(c, tat);
+    p->asmb.upload(c, tat);
+    RR_HIP(hipStreamSynchronize(c.stream));  // the host tables may go once the copies are done
+    return p;
+  } catch (...) {
+    if (p->work) (void)hipFree(p->work);
+    if (p->tables) (void)hipFree(p->tables);
+    delete p;
+    throw;
+  }
+}
+
+template <typename TI>
+void free_plan(BatchPlanT<TI>* p) {
+  if (!p) return;
+  if (p->work) (void)hipFree(p->work);
+  if (p->tables) (void)hipFree(p->tables);
+  delete p;
+}
+}  // namespace
+
+BatchPlan* batch_plan_create(Context& c, const std::vector<Tensor>& t, int prec, const std::vector<size_t>& r,
+                             const double* lin, const std::vector<size_t>& loff, double* jout, double* kout,
+                             const std::vector<size_t>& ooff) {
+  auto* bp = new BatchPlan();
+  bp->prec = prec;
+  try {
+    if (prec == P64) bp->p64 = build_plan<double>(c, t, prec, r, lin, loff, jout, kout, ooff);
+    else if (prec == P32) bp->p32 = build_plan<float>(c, t, prec, r, lin, loff, jout, kout, ooff);
+    else bp->p16 = build_plan<bf16_t>(c, t, prec, r, lin, loff, jout, kout, ooff);
+  } catch (...) {
+    delete bp;
+    throw;
+  }
+  return bp;
+}
+
+void batch_plan_enqueue(Context& c, const BatchPlan* bp) {
+  if (bp->p64) bp->p64->launch(c);
+  if (bp->p32) bp->p32->launch(c);
+  if (bp->p16) bp->p16->launch(c);
+  RR_HIP(hipGetLastError());
+}
+
+void batch_plan_free(BatchPlan* bp) {
+  if (!bp) return;
+  free_plan(bp->p64);
+  free_plan(bp->p32);
+  free_plan(bp->p16);
+  delete bp;
 }
 
 }  // namespace rr
