@@ -177,7 +177,7 @@ void mo_tensor_rkb(Context& c, int slot_ll, int slot_ss, size_t nl, size_t ns, c
 void mo_tensor_nonrel_embedded(Context& c, int slot, size_t n, const double* dC_spatial, double* dout);
 size_t slot_naux(Context& c, int slot);
 // A whole batch of factor-form J/K builds (one per tensor, D = L L^T) as a few grouped launches on
-// the context stream, with results identical to enqueue_jk_factor per tensor. L_m is read from
+// the context stream, with results identical to rr_ri_jk_factor (D = L L^T) per tensor. L_m is read from
 // lin + loff[m] (n x r row-major), J_m and K_m are written to jout / kout + ooff[m].
 struct BatchPlan;
 BatchPlan* batch_plan_create(Context& c, const std::vector<Tensor>& t, int prec, const std::vector<size_t>& r,
@@ -188,8 +188,6 @@ void batch_plan_free(BatchPlan* p);
 // Tensor storage outside the numbered slots (batches of molecules).
 void tensor_free(Tensor& t);
 void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t n, unsigned copies);
-// J and K of D = L L^T for one tensor, enqueued on the context stream without host synchronization.
-void enqueue_jk_factor(Context& c, const Tensor& t, int prec, const double* dL, size_t r, double* dJ, double* dK);
 // Switches the per-phase HIP-event timing on or off; returns the previous state.
 bool set_phase_timing(bool on);
 

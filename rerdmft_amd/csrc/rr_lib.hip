@@ -2555,30 +2555,6 @@ void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t
   t.naux_nz = (size_t)h_asym[1];
 }
 
-// J and K of D = L L^T (L device, n x r) on the context stream with no host synchronization: the
-// Coulomb term is taken from the exchange half-transform (gamma from W). Outputs device n x n.
-void enqueue_jk_factor(Context& c, const Tensor& t, int prec, const double* dL, size_t r, double* dJ, double* dK) {
-  const size_t n = t.n;
-  if (r == 0 || t.naux == 0) {
-    hipLaunchKernelGGL(k_zero, dim3(grid_for(n * n)), dim3(256), 0, c.stream, dJ, n * n);
-    hipLaunchKernelGGL(k_zero, dim3(grid_for(n * n)), dim3(256), 0, c.stream, dK, n * n);
-    RR_HIP(hipGetLastError());
-    return;
-  }
-  double* lt = static_cast<double*>(wsbuf(c, WS_T1, 2 * r * n * sizeof(double)));
-  hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(r * n)), dim3(256), 0, c.stream, dL, r, 1, r, r, n, n, 1.0, lt);
-  const size_t rp = roundup(r, 8);
-  double* dfac = static_cast<double*>(wsbuf(c, WS_DENS2, n * rp * sizeof(double)));
-  hipLaunchKernelGGL(k_build_dfac, dim3(grid_for(n * rp)), dim3(256), 0, c.stream, lt, (const double*)nullptr, r, n, n, rp,
-                     1, 1.0, 0.0, dfac);
-  RR_HIP(hipGetLastError());
-  double* gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
-  ExchangeOut ex = exchange_real_prec(c, t, prec, lt, nullptr, r, n, dfac, gamma);
-  double* jpart = nullptr;
-  const int jsplit = coulomb_from_gamma(c, t, prec, gamma, &jpart, t.symmetric);
-  real_assemble(c, t, jpart, jsplit, &ex, -0.5, nullptr, nullptr, dJ, dK, t.symmetric);
-}
-
 // ================================================================================================
 // Grouped launches: one launch runs a kernel body over many independent problems (segments), each
 // with its own arguments and block count; block b runs segment s (first[s] <= b < first[s+1]) as its
@@ -2727,7 +2703,7 @@ struct Grouped {
   }
 };
 
-// A batch of factor-form J/K builds (enqueue_jk_factor per molecule) as grouped launches; TI is the
+// A batch of factor-form J/K builds (rr_ri_jk_factor with D = L L^T per molecule) as grouped launches; TI is the
 // operand type. The per-molecule work buffers are carved from one allocation.
 template <typename TI>
 struct BatchPlanT {
