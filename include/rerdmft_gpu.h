@@ -309,6 +309,25 @@ int rr_mlp_loss_grad(rr_handle h, rr_mlp_t mlp, size_t batch, const double* x, c
 int rr_mlp_train_step(rr_handle h, rr_mlp_t mlp, size_t batch, const double* x, const double* target, double lr,
                       double* loss);
 
+/* ---- SCF DIIS on the device: scfCommutatorError and Diis<T>::extrapolate
+ * (Utils/DIIS.h:13-70, Utils/DIIS.cpp:51-138) ----------------------------------------------------
+ * rr_scf_commutator_error: e = F (P S) - (S P) F for row-major n x n real (is_complex = 0) or
+ * interleaved complex matrices. rr_diis_extrapolate stores (error, value) -- n_error / n_value doubles
+ * each, a complex entry counting twice -- keeping the newest max_vectors pairs in device memory, and
+ * writes sum_i w_i value_i with the weights minimizing |sum_i w_i e_i|^2 under sum_i w_i = 1 (B_ij =
+ * Re <e_i|e_j>, scaled by its largest diagonal; pivot ratio below 1e-14 drops the oldest pair);
+ * with fewer than two usable pairs, max_vectors < 2 or a zero error the value is copied unchanged and
+ * *n_weights = 0. weights (up to max_vectors doubles, newest first) and n_weights may be NULL.
+ * max_vectors is at most 16. Errors: "Diis::extrapolate: shapes differ from the stored history". */
+typedef struct rr_diis* rr_diis_t;
+int rr_scf_commutator_error(rr_handle h, int is_complex, size_t n, const double* fock, const double* density,
+                            const double* overlap, double* error);
+int rr_diis_create(rr_handle h, size_t max_vectors, rr_diis_t* out);
+int rr_diis_destroy(rr_handle h, rr_diis_t diis);
+int rr_diis_clear(rr_handle h, rr_diis_t diis);
+int rr_diis_extrapolate(rr_handle h, rr_diis_t diis, size_t n_error, const double* error, size_t n_value,
+                        const double* value, double* out, double* weights, size_t* n_weights);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,7 @@ EXPORTED = [
     "rr_batch_create", "rr_batch_destroy", "rr_batch_jk_factor",
     "rr_mlp_create", "rr_mlp_destroy", "rr_mlp_num_params", "rr_mlp_get_params", "rr_mlp_set_params",
     "rr_mlp_forward", "rr_mlp_loss_grad", "rr_mlp_train_step",
+    "rr_scf_commutator_error", "rr_diis_create", "rr_diis_destroy", "rr_diis_clear", "rr_diis_extrapolate",
 ]
 
 
@@ -99,6 +100,16 @@ class _Arr:
 
 def _f64(x):
     return _Arr(x, np.float64)
+
+
+def _is_complex(*xs):
+    for x in xs:
+        if type(x).__module__.startswith("torch"):
+            if x.is_complex():
+                return True
+        elif np.iscomplexobj(x):
+            return True
+    return False
 
 
 def _c128(x):
@@ -673,3 +684,39 @@ class Context:
         self._call("rr_mlp_train_step", mlp, _sz(x.shape[0]), xa.ptr, ta.ptr, ctypes.c_double(lr),
                    ctypes.byref(loss))
         return loss.value
+
+    # ---- SCF DIIS ------------------------------------------------------------------------------
+    def scf_commutator_error(self, fock, density, overlap, out=None):
+        """scfCommutatorError -- src/Utils/DIIS.cpp:51-58: F (P S) - (S P) F (real or complex)."""
+        cplx = _is_complex(fock, density, overlap)
+        conv = _c128 if cplx else _f64
+        f, p, s = conv(fock), conv(density), conv(overlap)
+        n = fock.shape[0]
+        e, ep = _out(out, (n, n), np.complex128 if cplx else np.float64)
+        self._call("rr_scf_commutator_error", ctypes.c_int(int(cplx)), _sz(n), f.ptr, p.ptr, s.ptr, ep)
+        return e
+
+    def diis_create(self, max_vectors):
+        """Diis<T>(max_vectors) -- src/Utils/DIIS.h:50-70, with its history in device memory."""
+        out = ctypes.c_void_p()
+        self._call("rr_diis_create", _sz(max_vectors), ctypes.byref(out))
+        return out
+
+    def diis_destroy(self, diis):
+        self._call("rr_diis_destroy", diis)
+
+    def diis_clear(self, diis):
+        self._call("rr_diis_clear", diis)
+
+    def diis_extrapolate(self, diis, error, value, out=None):
+        """Diis<T>::extrapolate -- src/Utils/DIIS.cpp:80-135. Returns (extrapolated value, weights)."""
+        cplx = _is_complex(value, error)
+        conv = _c128 if cplx else _f64
+        e, v = conv(error), conv(value)
+        cw = 2 if cplx else 1
+        o, op = _out(out, value.shape, np.complex128 if cplx else np.float64)
+        w = np.zeros(16)
+        nw = _sz()
+        self._call("rr_diis_extrapolate", diis, _sz(cw * int(np.prod(error.shape))), e.ptr,
+                   _sz(cw * int(np.prod(value.shape))), v.ptr, op, ctypes.c_void_p(w.ctypes.data), ctypes.byref(nw))
+        return o, w[:nw.value].copy()
