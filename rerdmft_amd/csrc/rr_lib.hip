@@ -1503,15 +1503,22 @@ long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool we
   const size_t nn = nx * nx * cw;
   double* X = static_cast<double*>(wsbuf(c, WS_T4 + 801, nn * sizeof(double)));
   double* T = static_cast<double*>(wsbuf(c, WS_T4 + 802, nn * sizeof(double)));
-  // spectral-norm bound from ||A^8||_F^(1/8) with A = M - I/2 (at most nx^(1/16) above ||A||_2)
+  // Scale of A = M - I/2 from its Frobenius powers: hi = ||A^8||_F^(1/8) >= ||A||_2 (at most
+  // nx^(1/16) above it), lo = (||A^8||_F / ||A^4||_F)^(1/4) <= ||A||_2 (Cauchy-Schwarz; equal to it
+  // when every eigenvalue of A has the same magnitude, as for an idempotent density). The start
+  // (M - I/2) / s with s = max(lo, 1.01 hi / sqrt 3) keeps every eigenvalue below sqrt 3 in
+  // magnitude, where the iteration converges to the right sign, and is exactly +-1 for an
+  // idempotent density.
   hipLaunchKernelGGL(k_shift_scale, dim3(grid_for(nn)), dim3(256), 0, c.stream, M, nx, cw, 0.5, 1.0, X);
   RR_HIP(hipGetLastError());
   dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // A^2
   dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, T, nx, T, nx, X, nx);  // A^4
+  const double a4 = std::sqrt(fro2(c, X, nx, cw, false));
   dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // A^8
   const double a8 = std::sqrt(fro2(c, T, nx, cw, false));
-  if (!(a8 > 0.0) || !std::isfinite(a8)) return -1;
-  const double s = std::pow(a8, 0.125);
+  if (!(a8 > 0.0) || !std::isfinite(a8) || !(a4 > 0.0) || !std::isfinite(a4)) return -1;
+  const double hi = std::pow(a8, 0.125), lo = std::pow(a8 / a4, 0.25);
+  const double s = std::max(lo, 1.01 * hi / std::sqrt(3.0));
   hipLaunchKernelGGL(k_shift_scale, dim3(grid_for(nn)), dim3(256), 0, c.stream, M, nx, cw, 0.5, 1.0 / s, X);
   RR_HIP(hipGetLastError());
   constexpr int kMaxSteps = 40;
