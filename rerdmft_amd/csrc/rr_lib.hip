@@ -1909,6 +1909,21 @@ __global__ void k_build_ct(const double* ct, size_t ldct, const double* occ, siz
   }
 }
 
+// out = rot(in) over interleaved bf16 (re, im) pairs, rot(re, im) = (im, -re): with it the imaginary
+// part of sum_k conj(a_k) b_k is the real dot product of a with rot(b) (count32 = number of pairs,
+// a multiple of 4).
+__global__ void k_rot_pairs_bf16(const bf16_t* in, bf16_t* out, size_t count32) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4* src = reinterpret_cast<const u4*>(in);
+  u4* dst = reinterpret_cast<u4*>(out);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count32 / 4; i += (size_t)gridDim.x * blockDim.x) {
+    u4 w = src[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = ((w[q] >> 16) | (w[q] << 16)) ^ 0x80000000u;
+    dst[i] = w;
+  }
+}
+
 struct Flavor {
   const Tensor* t;  // tensor of this flavor's spatial type
   size_t off;       // offset of the flavor in the coefficient rows (and in the output)
@@ -1971,25 +1986,50 @@ ExchangeOut exchange_flavors(Context& c, int prec, const std::vector<Flavor>& fl
   double* kpart = static_cast<double*>(wsbuf(c, WS_KPART, (size_t)(nsA + nsB) * nsq * 2 * sizeof(double)));
   {
     PhaseTimer pt(c, PH_GEMM2);
-    GemmArgs g;
-    g.A = W, g.lda = kdim, g.B = W, g.ldb = kdim;
-    g.C = kpart, g.ldcm = n_stack, g.ldcn = 1, g.sCz = nsq;
-    g.M = (int)n_stack, g.N = (int)n_stack, g.K = (int)kA, g.kper = (int)kperA, g.nsplit = nsA;
-    g.tri = 1, g.conj = 1;
-    GemmSpec sp;
-    sp.prec = prec, sp.out = P64, sp.cplx = true, sp.cfg = gemm2_cfg(c, n_stack);
-    gemm_auto(c, sp, g);
+    // bf16: the complex update as two real ones over the interleaved rows (the real matrix-core
+    // kernels are faster than the complex one): Re K = W . W, Im K = W . rot(W), written into the
+    // real and imaginary slots of the complex partial sums (column stride 2)
+    constexpr bool kReal = std::is_same<TI, bf16_t>::value;
+    const TI* Wrot = nullptr;
+    if constexpr (kReal) {
+      TI* wr = static_cast<TI*>(wsbuf(c, WS_TMP8 + 500, n_stack * ldw * sizeof(TI)));
+      hipLaunchKernelGGL(k_rot_pairs_bf16, dim3(grid_for(n_stack * kdim / 4)), dim3(256), 0, c.stream, W, wr, n_stack * kdim);
+      RR_HIP(hipGetLastError());
+      Wrot = wr;
+    }
+    // one part: rows [r0, r0 + m) of W, complex columns [k0, k0 + kd) into partials C (complex)
+    auto update = [&](size_t r0, size_t m, size_t k0, size_t kd, size_t kper, int ns, double* C) {
+      GemmArgs g;
+      g.tri = 1;
+      GemmSpec sp;
+      sp.prec = prec, sp.out = P64, sp.cfg = gemm2_cfg(c, m);
+      if constexpr (kReal) {
+        g.lda = ldw, g.ldb = ldw;
+        g.ldcm = 2 * n_stack, g.ldcn = 2, g.sCz = 2 * nsq;
+        g.M = (int)m, g.N = (int)m, g.K = (int)(2 * kd), g.kper = (int)(2 * kper), g.nsplit = ns;
+        sp.lds = gemm2_lds(c, m);
+        for (int part = 0; part < 2; ++part) {
+          g.A = W + r0 * ldw + 2 * k0;
+          g.B = (part ? Wrot : W) + r0 * ldw + 2 * k0;
+          g.C = C + part;
+          gemm_auto(c, sp, g);
+        }
+      } else {
+        g.A = W + r0 * ldw + 2 * k0, g.lda = kdim, g.B = W + r0 * ldw + 2 * k0, g.ldb = kdim;
+        g.C = C, g.ldcm = n_stack, g.ldcn = 1, g.sCz = nsq;
+        g.M = (int)m, g.N = (int)m, g.K = (int)kd, g.kper = (int)kper, g.nsplit = ns;
+        g.conj = 1;
+        sp.cplx = true;
+        gemm_auto(c, sp, g);
+      }
+    };
+    update(0, n_stack, 0, kA, kperA, nsA, kpart);
     if (kB) {
       // the other rows of these partial sums are zero
       double* pb = kpart + (size_t)nsA * nsq * 2;
       hipLaunchKernelGGL(k_zero, dim3(grid_for((size_t)nsB * nsq * 2)), dim3(256), 0, c.stream, pb, (size_t)nsB * nsq * 2);
       RR_HIP(hipGetLastError());
-      const TI* wb = W + row0[big] * ldw + 2 * kA;
-      g.A = wb, g.B = wb;
-      g.C = pb + 2 * (row0[big] * n_stack + row0[big]);
-      g.M = (int)rows[big], g.N = (int)rows[big], g.K = (int)kB, g.kper = (int)kperB, g.nsplit = nsB;
-      sp.cfg = gemm2_cfg(c, rows[big]);
-      gemm_auto(c, sp, g);
+      update(row0[big], rows[big], kA, kB, kperB, nsB, pb + 2 * (row0[big] * n_stack + row0[big]));
     }
     out.ktile = 1;
   }
