@@ -288,7 +288,7 @@ def main():
         "setup_s": setup_s,
         "batch_uneven_molecules": batch_leg(torch, rr, ctx, a.batch_molecules) if a.batch_molecules > 0 else None,
         "sigma_trial_densities": sigma_leg(torch, rr, ctx, a.sigma_densities) if a.sigma_densities > 0 else None,
-        "relativistic": relativistic_leg(torch, rr, ctx) if a.relativistic else None,
+        "relativistic": relativistic_leg(torch, rr, ctx, cpu=a.cpu_sample > 0) if a.relativistic else None,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -393,7 +393,50 @@ def _phases(torch, ctx, fn):
     return {k: round(v[0], 3) for k, v in ph.items() if v[1]}
 
 
-def relativistic_leg(torch, rr, ctx, reps=2):
+def relativistic_cpu(x2c_shape, rkb_shape):
+    """The oracle's X2C and RKB builds timed on the host over sampled aux rows (same sizes otherwise):
+    X2C at two sample sizes, extrapolated linearly to the full aux count (the occupied recovery is a
+    per-build constant); RKB (no recovery) scaled by n_aux / sample."""
+    from oracle import oracle as orc
+    from rerdmft_amd import synthetic as syn
+    orc.lib()
+    out = {}
+    nl, naux, npairs = x2c_shape
+    g = np.random.default_rng(SEED + 30)
+    q = syn.kramers_pair_occ(nl, npairs, SEED + 31)
+    p = q @ q.conj().T
+    eye = np.eye(2 * nl, dtype=np.complex128)
+    h = np.zeros((2 * nl, 2 * nl), dtype=np.complex128)
+    times = {}
+    for ns in (24, 96):
+        b = g.standard_normal((ns, nl * nl))
+        orc.x2c_fock_ri(b[:2], eye, eye, h, p)  # warm-up
+        t0 = time.perf_counter()
+        orc.x2c_fock_ri(b, eye, eye, h, p)
+        times[ns] = time.perf_counter() - t0
+    per_row = (times[96] - times[24]) / 72
+    t_full = times[24] + per_row * (naux - 24)
+    out["x2c"] = {"value": 1.0 / t_full, "unit": "Fock builds/s", "cores": orc.max_threads(), "kind": "port",
+                  "sample": f"x2cFockMatrix over 24 and 96 of {naux} aux rows ({times[24]:.2f} s, {times[96]:.2f} s), "
+                            f"extrapolated linearly to {naux}"}
+    nl, ns_, naux, npairs = rkb_shape
+    n = 2 * nl + 2 * ns_
+    nsamp = 48
+    bll = g.standard_normal((nsamp, nl * nl))
+    bss = g.standard_normal((nsamp, ns_ * ns_))
+    c, _ = np.linalg.qr(g.standard_normal((n, 2 * npairs)) + 1j * g.standard_normal((n, 2 * npairs)))
+    d = c @ c.conj().T
+    occ = np.ones(2 * npairs)
+    orc.ri_fock_two_electron_direct(nl, ns_, bll[:2], bss[:2], d, c, occ)  # warm-up
+    t0 = time.perf_counter()
+    orc.ri_fock_two_electron_direct(nl, ns_, bll, bss, d, c, occ)
+    dt = time.perf_counter() - t0
+    out["rkb"] = {"value": nsamp / (dt * naux), "unit": "Fock builds/s", "cores": orc.max_threads(), "kind": "port",
+                  "sample": f"riFockTwoElectronDirect over {nsamp} of {naux} aux rows ({dt:.2f} s), scaled by {naux}/{nsamp}"}
+    return out
+
+
+def relativistic_leg(torch, rr, ctx, reps=2, cpu=False):
     """X2C 2-spinor Fock build (n = 400 spatial, 800 spinors, n_aux = 2400, 80 Kramers pairs) and its
     AO->MO transform to 800 x 800 per P; 4c RKB riFockTwoElectronDirect (nL = 300, nS = 700, one shared
     n_aux = 4200 with the LL block non-zero on its first 1800 rows, 100 Kramers pairs). The AO tensors
@@ -490,6 +533,8 @@ def relativistic_leg(torch, rr, ctx, reps=2):
     ctx.release(ss)
     torch.cuda.empty_cache()
     out["rkb"] = rkb
+    if cpu:
+        out["cpu_baseline"] = relativistic_cpu((400, 2400, 80), (nl, ns, naux, npairs))
     return out
 
 
