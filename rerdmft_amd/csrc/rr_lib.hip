@@ -11,11 +11,13 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -861,11 +863,14 @@ struct PhaseTimer {
   Context& c;
   int phase;
   hipEvent_t a = nullptr, b = nullptr;
-  static bool enabled;
-  static int cur_prec;  // precision of the build in progress (set by the entry points)
+  // Shared by every handle: the switch is atomic, the precision of the build in progress is per
+  // calling thread, and the totals and the pending event pairs are guarded by `mu`.
+  static std::atomic<bool> enabled;
+  static thread_local int cur_prec;  // precision of the build in progress (set by the entry points)
   static double ms[24];
   static long long cnt[24];
   static std::vector<Pending> pending;
+  static std::mutex mu;
   PhaseTimer(Context& cc, int ph) : c(cc), phase(ph + 8 * cur_prec) {
     if (!enabled) return;
     RR_HIP(hipEventCreate(&a));
@@ -875,14 +880,19 @@ struct PhaseTimer {
   ~PhaseTimer() {
     if (!enabled || !a) return;
     if (hipEventRecord(b, c.stream) == hipSuccess) {
+      std::lock_guard<std::mutex> lock(mu);
       pending.push_back({phase, a, b});
-      if (pending.size() > 4096) resolve();
+      if (pending.size() > 4096) resolve_locked();
     } else {
       (void)hipEventDestroy(a);
       (void)hipEventDestroy(b);
     }
   }
   static void resolve() {
+    std::lock_guard<std::mutex> lock(mu);
+    resolve_locked();
+  }
+  static void resolve_locked() {
     for (const Pending& p : pending) {
       float t = 0.f;
       if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
@@ -895,11 +905,12 @@ struct PhaseTimer {
     pending.clear();
   }
 };
-bool PhaseTimer::enabled = false;
-int PhaseTimer::cur_prec = 0;
+std::atomic<bool> PhaseTimer::enabled{false};
+thread_local int PhaseTimer::cur_prec = 0;
 double PhaseTimer::ms[24] = {0};
 long long PhaseTimer::cnt[24] = {0};
 std::vector<PhaseTimer::Pending> PhaseTimer::pending;
+std::mutex PhaseTimer::mu;
 struct PrecScope {
   int saved;
   explicit PrecScope(int p) : saved(PhaseTimer::cur_prec) { PhaseTimer::cur_prec = (p >= 0 && p < 3) ? p : 0; }
@@ -3118,7 +3129,8 @@ int rr_set_recovery_method(rr_handle h, int method, long long* n_projector, long
 
 int rr_timing(rr_handle h, int enable, int reset, double* ms, long long* counts, int n) {
   if (!h) return RR_ERR_ARG;
-  rr::PhaseTimer::resolve();
+  std::lock_guard<std::mutex> lock(rr::PhaseTimer::mu);
+  rr::PhaseTimer::resolve_locked();
   for (int i = 0; i < n && i < 24; ++i) {
     if (ms) ms[i] = rr::PhaseTimer::ms[i];
     if (counts) counts[i] = rr::PhaseTimer::cnt[i];
