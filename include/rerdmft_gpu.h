@@ -328,6 +328,22 @@ int rr_diis_clear(rr_handle h, rr_diis_t diis);
 int rr_diis_extrapolate(rr_handle h, rr_diis_t diis, size_t n_error, const double* error, size_t n_value,
                         const double* value, double* out, double* weights, size_t* n_weights);
 
+/* ---- CASSCF RI contractions (MCSCF/CasFock.cpp:70-150, RI backends :244-262) ------------------
+ * b: the RI MO tensor (n_aux, n*n), row P = the n x n matrix B_P (Hermitian), real or interleaved
+ * complex (is_complex); <ab|cd> = sum_P B(P, a n + c) B(P, b n + d). Host or device arrays.
+ * rr_cas_two_electron_operator_ri: g (n x n) = G[rho]_rp = sum_st rho_st (<sr|tp> - <rs|tp>) for a
+ *   Hermitian rho (riTwoElectronOperator, CasFock.cpp:73-113; rho's eigenvalues |w| <= 1e-14 dropped).
+ * rr_cas_active_integrals_ri: out (a^2 x a^2) = (tu|vw) = sum_P B(P, act_t n + act_u) B(P, act_v n + act_w)
+ *   (riActiveIntegrals, :136-148).
+ * rr_cas_active_cumulant_fock_ri: f (n x a), F(p,q) = sum_r sum_P B(P, act_r n + p)
+ *   sum_(s,t) B(P, act_s n + act_t) lambda[((s a + t) a + r) a + q] (riActiveCumulantFock, :115-134). */
+int rr_cas_two_electron_operator_ri(rr_handle h, int is_complex, size_t n_aux, size_t n, const double* b,
+                                    const double* rho, double* g);
+int rr_cas_active_integrals_ri(rr_handle h, int is_complex, size_t n_aux, size_t n, const double* b, size_t n_act,
+                               const size_t* act, double* out);
+int rr_cas_active_cumulant_fock_ri(rr_handle h, int is_complex, size_t n_aux, size_t n, const double* b, size_t n_act,
+                                   const size_t* act, const double* lambda, double* f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,7 @@ EXPORTED = [
     "rr_mlp_create", "rr_mlp_destroy", "rr_mlp_num_params", "rr_mlp_get_params", "rr_mlp_set_params",
     "rr_mlp_forward", "rr_mlp_loss_grad", "rr_mlp_train_step",
     "rr_scf_commutator_error", "rr_diis_create", "rr_diis_destroy", "rr_diis_clear", "rr_diis_extrapolate",
+    "rr_cas_two_electron_operator_ri", "rr_cas_active_integrals_ri", "rr_cas_active_cumulant_fock_ri",
 ]
 
 
@@ -720,3 +721,38 @@ class Context:
         self._call("rr_diis_extrapolate", diis, _sz(cw * int(np.prod(error.shape))), e.ptr,
                    _sz(cw * int(np.prod(value.shape))), v.ptr, op, ctypes.c_void_p(w.ctypes.data), ctypes.byref(nw))
         return o, w[:nw.value].copy()
+
+    # ---- CASSCF RI contractions ----------------------------------------------------------------
+    def cas_two_electron_operator(self, b, rho, out=None):
+        """casTwoElectronOperator on an RI MO tensor -- src/MCSCF/CasFock.cpp:73-113, :246-259.
+        b: (n_aux, n, n) (or (n_aux, n*n)); rho: (n, n) Hermitian. Returns G (n, n)."""
+        cplx = _is_complex(b, rho)
+        conv = _c128 if cplx else _f64
+        n = rho.shape[0]
+        naux = b.shape[0]
+        bb, rr_ = conv(b), conv(rho)
+        g, gp = _out(out, (n, n), np.complex128 if cplx else np.float64)
+        self._call("rr_cas_two_electron_operator_ri", ctypes.c_int(int(cplx)), _sz(naux), _sz(n), bb.ptr, rr_.ptr, gp)
+        return g
+
+    def cas_active_integrals(self, b, n, active):
+        """casActiveIntegrals -- src/MCSCF/CasFock.cpp:136-148: (tu|vw) as an (a^2, a^2) array."""
+        cplx = _is_complex(b)
+        bb = (_c128 if cplx else _f64)(b)
+        act = (ctypes.c_size_t * max(1, len(active)))(*active)
+        a2 = len(active) ** 2
+        out = np.empty((a2, a2), dtype=np.complex128 if cplx else np.float64)
+        self._call("rr_cas_active_integrals_ri", ctypes.c_int(int(cplx)), _sz(b.shape[0]), _sz(n), bb.ptr,
+                   _sz(len(active)), act, ctypes.c_void_p(out.ctypes.data))
+        return out
+
+    def cas_active_cumulant_fock(self, b, n, active, lam):
+        """casActiveCumulantFock -- src/MCSCF/CasFock.cpp:115-134: F (n, a)."""
+        cplx = _is_complex(b, lam)
+        conv = _c128 if cplx else _f64
+        bb, ll = conv(b), conv(lam)
+        act = (ctypes.c_size_t * max(1, len(active)))(*active)
+        f = np.empty((n, len(active)), dtype=np.complex128 if cplx else np.float64)
+        self._call("rr_cas_active_cumulant_fock_ri", ctypes.c_int(int(cplx)), _sz(b.shape[0]), _sz(n), bb.ptr,
+                   _sz(len(active)), act, ll.ptr, ctypes.c_void_p(f.ctypes.data))
+        return f

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librerdmft_gpu.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rr_lib.hip", "rr_sigma.hip", "rr_transform.hip", "rr_rkb_chol.hip",
-                                                       "rr_batch.hip", "rr_mlp.hip", "rr_diis.hip", "rr_gemm_wide.hip")]
+                                                       "rr_batch.hip", "rr_mlp.hip", "rr_diis.hip", "rr_cas.hip", "rr_gemm_wide.hip")]
 # translation units whose matrix-core accumulators stay in the AGPR form (wide tiles)
 AGPR_FORM = {"rr_gemm_wide.hip"}
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("rr_device.hpp", "rr_gemm.hpp", "rr_core.hpp")] + [

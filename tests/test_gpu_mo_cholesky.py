@@ -129,3 +129,19 @@ def test_cholesky_mo_vectors(ctx, oracle):
     cc = np.array([[complex(rng.next(), rng.next()) for _ in range(nmo)] for _ in range(2 * n)])
     ws = ctx.cholesky_to_mo_spinor(6, cc)
     assert rel(ws, oracle.cholesky_to_mo_spinor(vec.reshape(nchol, -1), cc)) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [(rr.FP64, 1e-12), (rr.FP32, 2e-5), (rr.BF16, 3e-2)])
+def test_cholesky_factor_form_equals_density_form(ctx, oracle, prec, tol):
+    # AoCholesky.cpp:133-136 builds K = sum_L B_L P B_L from the density; for P = 2 C C^T the same K is
+    # sum_L (B_L sqrt2 C)(B_L sqrt2 C)^T, the factor-form exchange with N_chol in place of n_aux
+    n, nchol, nocc = 64, 150, 9
+    vec = syn.ri_tensor(nchol, n, 0.3, 10.0, 761)
+    c = syn.orthonormal_occ(n, nocc, 762)
+    p = 2.0 * c @ c.T
+    h = syn.symmetric(n, 763)
+    ctx.upload(7, vec, rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16)
+    f_ref = oracle.nonrel_fock_cholesky(vec.reshape(nchol, -1), h, p)
+    assert rel(ctx.nonrel_fock_cholesky(7, h, p), f_ref) < 1e-12
+    j, k = ctx.ri_jk_factor(7, None, np.sqrt(2.0) * c, precision=prec)
+    assert rel(h + j - 0.5 * k, f_ref) < tol
