@@ -344,6 +344,14 @@ int rr_cas_active_integrals_ri(rr_handle h, int is_complex, size_t n_aux, size_t
 int rr_cas_active_cumulant_fock_ri(rr_handle h, int is_complex, size_t n_aux, size_t n, const double* b, size_t n_act,
                                    const size_t* act, const double* lambda, double* f);
 
+/* UCCSD integrals of an RI MO tensor (CC/Uccsd.cpp:128-180, makeCcIntegrals(RiMoEri) :181-184):
+ * out[p][q][r][s] = <P_p Q_q | R_r S_s> = sum_aux B(aux, P_p n + R_r) B(aux, Q_q n + S_s) for the index
+ * lists p (np), q (nq), r (nr), s (ns) (physicsBlock; slab(p, list) is the block ({p}, list, list,
+ * list)). The NON_REL spin-orbital source (makeCcIntegrals(RiNonRelSpinMoEri)) is the same call on
+ * the spin-embedded tensor of rr_ri_nonrel_spin_embedded_b. */
+int rr_cc_physics_block(rr_handle h, int is_complex, size_t n_aux, size_t n, const double* b, size_t np, const size_t* p,
+                        size_t nq, const size_t* q, size_t nr, const size_t* r, size_t ns, const size_t* s, double* out);
+
 #ifdef __cplusplus
 }
 #endif

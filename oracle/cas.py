@@ -54,3 +54,13 @@ def active_cumulant_fock(b, n, act, lam, planted=False):
     for r in range(a):
         f += b[:, act[r], :].T @ tm[:, r * a:(r + 1) * a]
     return f
+
+
+def physics_block(b, n, p, q, r, s):
+    """FactoredCcIntegrals::physicsBlock with the dense pair factor (src/CC/Uccsd.cpp:128-153,
+    DensePairFactor :27-45): A[P][(p,r)], B'[P][(q,s)] gathered from B, x = A^T B', out[p][q][r][s]."""
+    b = b.reshape(b.shape[0], n, n)
+    a = b[:, p][:, :, r].reshape(b.shape[0], -1)
+    bk = b[:, q][:, :, s].reshape(b.shape[0], -1)
+    x = (a.T @ bk).reshape(len(p), len(r), len(q), len(s))
+    return x.transpose(0, 2, 1, 3)

@@ -41,6 +41,7 @@ EXPORTED = [
     "rr_mlp_forward", "rr_mlp_loss_grad", "rr_mlp_train_step",
     "rr_scf_commutator_error", "rr_diis_create", "rr_diis_destroy", "rr_diis_clear", "rr_diis_extrapolate",
     "rr_cas_two_electron_operator_ri", "rr_cas_active_integrals_ri", "rr_cas_active_cumulant_fock_ri",
+    "rr_cc_physics_block",
 ]
 
 
@@ -756,3 +757,15 @@ class Context:
         self._call("rr_cas_active_cumulant_fock_ri", ctypes.c_int(int(cplx)), _sz(b.shape[0]), _sz(n), bb.ptr,
                    _sz(len(active)), act, ll.ptr, ctypes.c_void_p(f.ctypes.data))
         return f
+
+    def cc_physics_block(self, b, n, p, q, r, s):
+        """FactoredCcIntegrals::physicsBlock over an RI MO tensor -- src/CC/Uccsd.cpp:128-153:
+        out[p][q][r][s] = sum_aux B(aux, p n + r) B(aux, q n + s) for the index lists."""
+        cplx = _is_complex(b)
+        bb = (_c128 if cplx else _f64)(b)
+        lists = [(ctypes.c_size_t * max(1, len(x)))(*x) for x in (p, q, r, s)]
+        out = np.empty((len(p), len(q), len(r), len(s)), dtype=np.complex128 if cplx else np.float64)
+        self._call("rr_cc_physics_block", ctypes.c_int(int(cplx)), _sz(b.shape[0]), _sz(n), bb.ptr,
+                   _sz(len(p)), lists[0], _sz(len(q)), lists[1], _sz(len(r)), lists[2], _sz(len(s)), lists[3],
+                   ctypes.c_void_p(out.ctypes.data))
+        return out

@@ -128,13 +128,35 @@ __global__ void k_cas_gather_fock(const double* b, const double* tm, const long 
   }
 }
 
-const long long* upload_act(Context& c, size_t a, const size_t* act, size_t n) {
+// out[P][(i,j)] = B[P][I[i] n + J[j]]
+__global__ void k_cc_gather_pairs(const double* b, const long long* I, size_t ni, const long long* J, size_t nj,
+                                  size_t naux, size_t n, int cw, double* out) {
+  const size_t m = ni * nj;
+  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < naux * m; x += (size_t)gridDim.x * blockDim.x) {
+    const size_t P = x / m, ij = x % m, i = ij / nj, j = ij % nj;
+    const size_t src = P * n * n + (size_t)I[i] * n + (size_t)J[j];
+    for (int q = 0; q < cw; ++q) out[x * cw + q] = b[src * cw + q];
+  }
+}
+
+// out[p][q][r][s] = x[(p r)][(q s)]
+__global__ void k_cc_permute(const double* x, size_t np, size_t nq, size_t nr, size_t ns, int cw, double* out) {
+  const size_t total = np * nq * nr * ns;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total; o += (size_t)gridDim.x * blockDim.x) {
+    const size_t s_ = o % ns, r = (o / ns) % nr, q = (o / (ns * nr)) % nq, p = o / (ns * nr * nq);
+    const size_t src = (p * nr + r) * (nq * ns) + q * ns + s_;
+    for (int k = 0; k < cw; ++k) out[o * cw + k] = x[src * cw + k];
+  }
+}
+
+const long long* upload_act(Context& c, size_t a, const size_t* act, size_t n, int slot = 0,
+                            const char* what = "CAS: active index out of range") {
   std::vector<long long> h(a);
   for (size_t i = 0; i < a; ++i) {
-    RR_REQUIRE(act[i] < n, RR_ERR_ARG, "CAS: active index out of range");
+    RR_REQUIRE(act[i] < n, RR_ERR_ARG, what);
     h[i] = (long long)act[i];
   }
-  long long* d = static_cast<long long*>(rr::wsbuf(c, WS_CS_IDX, std::max<size_t>(a, 1) * sizeof(long long)));
+  long long* d = static_cast<long long*>(rr::wsbuf(c, WS_CS_IDX + 1000 * slot, std::max<size_t>(a, 1) * sizeof(long long)));
   RR_HIP(hipMemcpyAsync(d, h.data(), a * sizeof(long long), hipMemcpyHostToDevice, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));  // h is released on return
   return d;
@@ -277,6 +299,39 @@ int rr_cas_active_cumulant_fock_ri(rr_handle h, int is_complex, size_t n_aux, si
     RR_HIP(hipGetLastError());
     rr::dgemm_small(c, cx, 'T', 'N', n, a, a * n_aux, x, n, yt, a, of.dev, a);  // F = X^T Yt
     rr::finish_out(c, of);
+  });
+}
+
+int rr_cc_physics_block(rr_handle h, int is_complex, size_t n_aux, size_t n, const double* b, size_t np, const size_t* p,
+                        size_t nq, const size_t* q, size_t nr, const size_t* r, size_t ns, const size_t* s, double* out) {
+  return guarded(h, [&](Context& c) {
+    RR_REQUIRE(b && out, RR_ERR_ARG, "FactoredCcIntegrals::physicsBlock: null input");
+    const int cw = is_complex ? 2 : 1;
+    const size_t M = np * nr, N = nq * ns;
+    if (M == 0 || N == 0) return;
+    const char* msg = "FactoredCcIntegrals::physicsBlock: index out of range";
+    const long long* dp = upload_act(c, np, p, n, 1, msg);
+    const long long* dq = upload_act(c, nq, q, n, 2, msg);
+    const long long* drr = upload_act(c, nr, r, n, 3, msg);
+    const long long* ds = upload_act(c, ns, s, n, 4, msg);
+    const double* dB = rr::in_buf(c, WS_CS_B, b, n_aux * n * n * cw);
+    rr::OutBuf oo = rr::out_buf(c, WS_CS_OUT, out, M * N * cw);
+    double* x = static_cast<double*>(rr::wsbuf(c, WS_CS_X, M * N * cw * sizeof(double)));
+    if (n_aux == 0) {
+      hipLaunchKernelGGL(k_cas_zero, dim3(rr::grid_for(M * N * cw)), dim3(256), 0, c.stream, x, M * N * cw);
+    } else {
+      double* ga = static_cast<double*>(rr::wsbuf(c, WS_CS_BA, n_aux * M * cw * sizeof(double)));
+      double* gb = static_cast<double*>(rr::wsbuf(c, WS_CS_YT, n_aux * N * cw * sizeof(double)));
+      hipLaunchKernelGGL(k_cc_gather_pairs, dim3(rr::grid_for(n_aux * M)), dim3(256), 0, c.stream, dB, dp, np, drr, nr,
+                         n_aux, n, cw, ga);
+      hipLaunchKernelGGL(k_cc_gather_pairs, dim3(rr::grid_for(n_aux * N)), dim3(256), 0, c.stream, dB, dq, nq, ds, ns,
+                         n_aux, n, cw, gb);
+      RR_HIP(hipGetLastError());
+      rr::dgemm_small(c, is_complex != 0, 'T', 'N', M, N, n_aux, ga, M, gb, N, x, N);  // x = A^T B (no conjugation)
+    }
+    hipLaunchKernelGGL(k_cc_permute, dim3(rr::grid_for(M * N)), dim3(256), 0, c.stream, x, np, nq, nr, ns, cw, oo.dev);
+    RR_HIP(hipGetLastError());
+    rr::finish_out(c, oo);
   });
 }
 

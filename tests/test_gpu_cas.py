@@ -80,3 +80,16 @@ def test_rejects_bad_active_index(ctx):
     b = _tensor(4, 5, 9, False)
     with pytest.raises(rr.GpuError, match="active index out of range"):
         ctx.cas_active_integrals(b, 5, [0, 5])
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_cc_physics_block(ctx, cplx):
+    naux, n = 40, 30
+    b = _tensor(naux, n, 13, cplx)
+    p, q, r, s = [0, 7, 29], list(range(5, 20)), [3, 4], list(range(30))
+    got = ctx.cc_physics_block(b, n, p, q, r, s)
+    ref = cas.physics_block(b, n, p, q, r, s)
+    assert got.shape == (3, 15, 2, 30)
+    assert rel(got, ref) < 1e-12
+    with pytest.raises(rr.GpuError, match="index out of range"):
+        ctx.cc_physics_block(b, n, [30], q, r, s)

@@ -74,3 +74,16 @@ def test_active_cumulant_fock_matches_dense(cplx):
     np.testing.assert_allclose(got, ref, atol=1e-11)
     bad = cas.active_cumulant_fock(b, n, act, lam.reshape(-1), planted=True)
     assert np.max(np.abs(bad - ref)) > 1e-3
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_physics_block_matches_dense(cplx):
+    n, naux = 9, 6
+    b = _tensor(naux, n, 11, cplx)
+    eri = _dense(b)  # <ab|cd> = sum_P B(P, a n + c) B(P, b n + d)
+    p, q, r, s = [0, 4], [1, 2, 8], [3], [5, 6]
+    ref = eri[np.ix_(p, q, r, s)]
+    np.testing.assert_allclose(cas.physics_block(b, n, p, q, r, s), ref, atol=1e-12)
+    # slab(p, list) is the block ({p}, list, list, list)
+    lst = [1, 3, 7]
+    np.testing.assert_allclose(cas.physics_block(b, n, [2], lst, lst, lst)[0], eri[2][np.ix_(lst, lst, lst)], atol=1e-12)
