@@ -71,3 +71,27 @@ def test_batch_rebuilds_graph_when_ranks_change(ctx):
         assert np.max(np.abs(ks[0] - kr)) < 1e-11 * np.max(np.abs(kr))
     finally:
         ctx.batch_destroy(batch)
+
+
+@pytest.mark.parametrize("prec", [rr.FP64, rr.FP32])
+def test_batch_ragged_sizes_and_empty_factor(ctx, prec):
+    # tiny and ragged orbital counts, and a molecule with no occupied orbitals (J = K = 0)
+    sizes, ranks = [3, 8, 17, 33, 12], [1, 0, 4, 7, 12]
+    mols = [syn.ri_tensor(max(1, 4 * n - 5), n, 0.3, 5.0, 900 + n) for n in sizes]
+    copies = rr.COPY_FP64 | rr.COPY_FP32
+    batch = ctx.batch_create(mols, copies)
+    try:
+        g = np.random.default_rng(901)
+        ls = [g.standard_normal((n, r)) for n, r in zip(sizes, ranks)]
+        for call in range(2):
+            js, ks, rep = ctx.batch_jk_factor(batch, ls, precision=prec)
+            assert rep == (call > 0)
+            for b, l, j, k in zip(mols, ls, js, ks):
+                if l.shape[1] == 0:
+                    assert not np.any(j) and not np.any(k)
+                    continue
+                ctx.upload(13, b, copies)
+                j1, k1 = ctx.ri_jk_factor(13, None, l, precision=prec)
+                assert np.array_equal(j, j1) and np.array_equal(k, k1)
+    finally:
+        ctx.batch_destroy(batch)
