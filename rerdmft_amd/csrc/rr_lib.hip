@@ -1017,7 +1017,9 @@ int gemm2_lds(Context& c, size_t n) {
 int gemm2_splits(size_t n, size_t kdim, int slab) {
   const size_t t = (n + 127) / 128;
   const size_t tiles = std::max<size_t>(1, t * (t + 1) / 2);
-  size_t ns = std::max<size_t>(1, (2048 + tiles - 1) / tiles);
+  // at most 2048 workgroups (eight full rounds of one workgroup per CU): a count just above a
+  // multiple of the CU count leaves a nearly empty last round of long-running tiles
+  size_t ns = std::max<size_t>(1, 2048 / tiles);
   ns = std::min<size_t>(ns, std::max<size_t>(1, kdim / (size_t)(slab * 8)));
   return (int)std::min<size_t>(ns, 256);
 }
