@@ -19,6 +19,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/rerdmft_gpu.h"
@@ -165,6 +166,59 @@ CplxMat buildRiMoEri(const GpuRiDirectEriSource<RealMat, CplxMat>& eri, const Cp
                                     detail::dptr(c_ukb), nt, out.data()));
   return out;
 }
+
+// ---- DIIS: counterpart of Diis<T> and scfCommutatorError (Utils/DIIS.h:45-70), with the
+// (error, Fock) history kept in device memory ------------------------------------------------------
+namespace detail {
+template <class Mat>
+constexpr std::size_t scalar_doubles() {
+  using T = typename std::remove_cv<typename std::remove_pointer<decltype(std::declval<const Mat&>().data())>::type>::type;
+  return sizeof(T) / sizeof(double);  // 1 for double, 2 for std::complex<double>
+}
+}  // namespace detail
+
+template <class Mat>
+Mat scfCommutatorError(Device& dev, const Mat& fock, const Mat& density, const Mat& overlap) {
+  Mat e(fock.rows(), fock.cols());
+  dev.check(rr_scf_commutator_error(dev.handle(), detail::scalar_doubles<Mat>() == 2, fock.rows(), detail::dptr(fock),
+                                    detail::dptr(density), detail::dptr(overlap), detail::dptr(e)));
+  return e;
+}
+
+template <class Mat>
+class GpuDiis {
+ public:
+  GpuDiis(std::shared_ptr<Device> dev, std::size_t max_vectors) : dev_(std::move(dev)), max_(max_vectors) {
+    dev_->check(rr_diis_create(dev_->handle(), max_vectors, &d_));
+  }
+  ~GpuDiis() {
+    if (d_) rr_diis_destroy(dev_->handle(), d_);
+  }
+  GpuDiis(const GpuDiis&) = delete;
+  GpuDiis& operator=(const GpuDiis&) = delete;
+  Mat extrapolate(const Mat& error, const Mat& value) {
+    constexpr std::size_t k = detail::scalar_doubles<Mat>();
+    Mat out(value.rows(), value.cols());
+    double w[16];
+    std::size_t nw = 0;
+    dev_->check(rr_diis_extrapolate(dev_->handle(), d_, k * error.rows() * error.cols(), detail::dptr(error),
+                                    k * value.rows() * value.cols(), detail::dptr(value), detail::dptr(out), w, &nw));
+    last_.assign(w, w + nw);
+    return out;
+  }
+  void clear() {
+    dev_->check(rr_diis_clear(dev_->handle(), d_));
+    last_.clear();
+  }
+  std::size_t maxVectors() const { return max_; }
+  const std::vector<double>& lastWeights() const { return last_; }
+
+ private:
+  std::shared_ptr<Device> dev_;
+  rr_diis_t d_ = nullptr;
+  std::size_t max_;
+  std::vector<double> last_;
+};
 
 }  // namespace gpu
 }  // namespace rerdmft

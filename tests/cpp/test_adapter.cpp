@@ -146,6 +146,41 @@ double scf(const M& h, const Eri& eri, const M& x, std::size_t nocc, int* iters)
   return e;
 }
 
+// The same loop with DIIS on the Fock matrix (NonRelHartreeFock.cpp:118-198 with its default DIIS):
+// the commutator error and the extrapolation run through the GPU counterparts of the reference's
+// scfCommutatorError / Diis<double>.
+template <class Eri>
+double scf_diis(const M& h, const Eri& eri, const M& x, const M& s, std::size_t nocc,
+                const std::shared_ptr<rerdmft::gpu::Device>& dev, int* iters) {
+  const std::size_t n = h.rows();
+  auto density = [&](const M& f) {
+    std::vector<double> w;
+    M y;
+    eigh(mul(x, mul(f, x), true), w, y);
+    M c = mul(x, y);
+    M p(n, n);
+    for (std::size_t a = 0; a < n; ++a)
+      for (std::size_t b = 0; b < n; ++b)
+        for (std::size_t i = 0; i < nocc; ++i) p(a, b) += 2.0 * c(a, i) * c(b, i);
+    return p;
+  };
+  rerdmft::gpu::GpuDiis<M> diis(dev, 5);
+  M p = density(h);
+  double e_old = 0.0, e = 0.0;
+  for (int it = 0; it < 200; ++it) {
+    M f = nonRelFockMatrix(h, eri, p);
+    e = 0.0;
+    for (std::size_t a = 0; a < n; ++a)
+      for (std::size_t b = 0; b < n; ++b) e += 0.5 * p(a, b) * (h(a, b) + f(a, b));
+    *iters = it + 1;
+    if (it > 0 && std::fabs(e - e_old) < 1e-12) break;
+    e_old = e;
+    const M err = rerdmft::gpu::scfCommutatorError(*dev, f, p, s);
+    p = density(diis.extrapolate(err, f));
+  }
+  return e;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s <liboracle.so>\n", argv[0]);
@@ -190,6 +225,17 @@ int main(int argc, char** argv) {
   std::printf("E_gpu = %.12f (%d iterations)\nE_cpu = %.12f (%d iterations)\n|dE| = %.3e\n", e_gpu, it_gpu, e_cpu, it_cpu,
               std::fabs(e_gpu - e_cpu));
   bool ok = std::fabs(e_gpu - e_cpu) < 1e-10;
+  // DIIS through the GPU counterparts, on a weaker two-electron part where the SCF converges: the
+  // energy must equal the (slower) linearly mixed loop's
+  M b2 = b;
+  for (std::size_t i = 0; i < naux * n * n; ++i) b2.data()[i] *= 0.25;
+  rerdmft::gpu::GpuRiNonRelEriSource<M> gsrc2(dev, n, b2, x, s, RR_FP64);
+  int it_mix = 0, it_diis = 0;
+  const double e_mix = scf(h, gsrc2, x, nocc, &it_mix);
+  const double e_diis = scf_diis(h, gsrc2, x, s, nocc, dev, &it_diis);
+  std::printf("weaker system: E_mix = %.12f (%d iterations), E_diis = %.12f (%d iterations)\n", e_mix, it_mix, e_diis,
+              it_diis);
+  ok = ok && it_mix < 200 && it_diis < it_mix && std::fabs(e_diis - e_mix) < 1e-9;
   // shape error propagates as the reference's message
   try {
     M wrong(n + 1, n + 1);

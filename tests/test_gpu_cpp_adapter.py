@@ -1,5 +1,7 @@
 """The C++ adapter (rerdmft_amd/host/rerdmft_gpu.hpp) as a drop-in: a templated RHF SCF loop run with
-the GPU ERI source and with the CPU oracle must converge to the same energy (1e-10 Ha)."""
+the GPU ERI source and with the CPU oracle must reach the same energy (1e-10 Ha); the same loop with
+the adapter's GpuDiis / scfCommutatorError must converge to the linearly mixed loop's energy in fewer
+iterations."""
 import os
 import subprocess
 
@@ -19,3 +21,4 @@ def test_cpp_adapter_scf_matches_oracle():
     assert r.returncode == 0, r.stdout + r.stderr
     assert "[PASS]" in r.stdout
     assert "density_matrix dimension does not match nl" in r.stdout
+    assert "E_diis" in r.stdout
