@@ -1020,7 +1020,8 @@ int gemm2_splits(size_t n, size_t kdim, int slab) {
   // at most 2048 workgroups (eight full rounds of one workgroup per CU): a count just above a
   // multiple of the CU count leaves a nearly empty last round of long-running tiles
   size_t ns = std::max<size_t>(1, 2048 / tiles);
-  ns = std::min<size_t>(ns, std::max<size_t>(1, kdim / (size_t)(slab * 8)));
+  // at least 256 slabs of k per split: small products (the molecules of a batch) keep long k-loops
+  ns = std::min<size_t>(ns, std::max<size_t>(1, kdim / (size_t)(slab * 256)));
   return (int)std::min<size_t>(ns, 256);
 }
 
