@@ -540,9 +540,7 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               // bf16 through the eight-wave full-line kernel (two matrix-core steps per stage): 34-36
                               {256, 208, 8, 1}, {256, 224, 4, 1}, {128, 208, 8, 1},
                               // fp64 / fp32 eight-wave square tiles for the rank-k update: 37-38
-                              {128, 128, 4, 0, 1}, {192, 192, 4, 0, 1},
-                              // lean bf16 direct-to-LDS, eight waves of 32 x 112 (two workgroups per CU): 39
-                              {128, 224, 4, 1}};
+                              {128, 128, 4, 0, 1}, {192, 192, 4, 0, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 // rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
@@ -596,7 +594,6 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 22: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 128, 2, 4>), dim3(grid), dim3(256), 0, s, g); return;
           case 25: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 224, 2, 3>), dim3(grid), dim3(256), 0, s, g); return;
           case 30: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 208, 8, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
-          case 39: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 128, 224, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 31: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 208, 8, 4, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 32: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 224, 4, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 33: hipLaunchKernelGGL((gemm_nt_dma2_kernel<TO, 256, 224, 4, 4, 8>), dim3(grid), dim3(512), 0, s, g); return;

@@ -177,7 +177,7 @@ def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
     ctx.upload(9, b, rr.COPY_BF16)
     outs = []
     try:
-        for cfg in [1, 2, 5, 100, 101] + list(range(102, 126)) + list(range(130, 137)) + [139]:
+        for cfg in [1, 2, 5, 100, 101] + list(range(102, 126)) + list(range(130, 137)):
             ctx.set_tile_config(cfg)
             dj, dk = ctx.ri_jk_delta(9, c_new, c_old, precision=rr.BF16)
             j, k = ctx.ri_jk_factor(9, None, l, precision=rr.BF16)
