@@ -44,6 +44,11 @@ struct GemmArgs {
   // sAk (sBk) logical elements after block 0. kper then counts flat k over kbatch * K.
   int kbatch = 0;
   long long sAk = 0, sBk = 0;
+  // Row blocks of the output (gemm_nt_lds_kernel only): with mrow > 0, row m is written as row
+  // m % mrow of block m / mrow, blocks sCmb logical elements apart -- a stack of products that share
+  // the B operand computed as one tall product.
+  int mrow = 0;
+  long long sCmb = 0;
   // 16+ bytes of zeros in device memory: the source of out-of-range chunks in the direct-to-LDS kernel
   const void* zpage = nullptr;
 };
@@ -449,6 +454,11 @@ __device__ __forceinline__ void gemm_nt_lds_tile(const GemmArgs& g, int w) {
           const int b = nn / g.nblk;
           nn -= b * g.nblk;
           base += static_cast<long long>(b) * g.sCnb;
+        }
+        if (g.mrow > 0) {
+          const int b = mm / g.mrow;
+          mm -= b * g.mrow;
+          base += static_cast<long long>(b) * g.sCmb;
         }
         TO* p = base + static_cast<long long>(mm) * g.ldcm + static_cast<long long>(nn) * g.ldcn;
         double v = g.alpha * static_cast<double>(acc[i][j][e]);

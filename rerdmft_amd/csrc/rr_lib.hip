@@ -2772,7 +2772,10 @@ struct BatchPlanT {
   Grouped<OpDfac> dfac;
   Grouped<OpPack<TI>> pack_t;
   // half-transform: 64 x 48 tiles (N = padded factor rank, at most a few tens here)
-  Grouped<OpGemmLds<TI, TI, 64, 48, 4 * Mma<TI>::KL * (std::is_same<TI, bf16_t>::value ? 2 : 1), 4>> gemm1;
+  // (64 x 16, 64 x 32, 128 x 16/32/48 measured slower on the 256-molecule batch: narrower tiles
+  // re-read the tensor rows once per column tile)
+  static constexpr int G1BM = 64, G1BN = 48;
+  Grouped<OpGemmLds<TI, TI, G1BM, G1BN, 4 * Mma<TI>::KL * (std::is_same<TI, bf16_t>::value ? 2 : 1), 4>> gemm1;
   Grouped<OpGammaW<TI>> gam;
   Grouped<OpGammaReduce> gam_red;
   // rank-k update: 64 x 64 lower tiles
@@ -2898,10 +2901,14 @@ BatchPlanT<TI>* build_plan(Context& c, const std::vector<Tensor>& ts, int prec, 
         a.A = tensor_data(t, prec), a.lda = ld, a.sAb = n * ld;
         a.B = ltT, a.ldb = ld, a.sBb = 0;
         a.C = W, a.ldcm = kdim, a.ldcn = 1, a.sCb = rp;
-        a.M = (int)n, a.N = (int)rp, a.K = (int)roundup(n, 8), a.nbatch = (int)naux;
-        a.tiles_m = (a.M + 63) / 64, a.tiles_n = (a.N + 47) / 48, a.ntiles = a.tiles_m * a.tiles_n;
+        // all aux rows of the molecule as one tall product (rows (P, nu) are contiguous in the
+        // tensor), each row written to row nu of W's column block P
+        a.M = (int)(naux * n), a.N = (int)rp, a.K = (int)roundup(n, 8), a.nbatch = 1, a.sAb = 0, a.sCb = 0;
+        a.mrow = (int)n, a.sCmb = (long long)rp;
+        a.tiles_m = (a.M + BatchPlanT<TI>::G1BM - 1) / BatchPlanT<TI>::G1BM;
+        a.tiles_n = (a.N + BatchPlanT<TI>::G1BN - 1) / BatchPlanT<TI>::G1BN, a.ntiles = a.tiles_m * a.tiles_n;
         a.kper = a.K, a.nsplit = 1, a.xcd_remap = 0, a.zpage = c.zpage;
-        p->gemm1.add(a, (size_t)a.ntiles * naux);
+        p->gemm1.add(a, (size_t)a.ntiles);
       }
       int gsplit;
       {  // gamma from W (gamma_from_w)
