@@ -262,7 +262,29 @@ __device__ __forceinline__ void k_coulomb_body(const TB* B, size_t rowlen, size_
   bool valid[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) valid[u] = base + (size_t)u * 256 * VN < rowlen;
-  for (size_t P = pb; P < pe; ++P) {
+  // FP64: two aux rows per iteration, both rows' loads issued before either row is accumulated (the
+  // sum still runs over P in order); narrower types keep one row (the wider rows cost occupancy)
+  size_t P = pb;
+  if constexpr (sizeof(TB) == 8)
+  for (; P + 1 < pe; P += 2) {
+    double bv[2][U][VN];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (valid[u]) V::load(B + (P + h) * rowlen + base + (size_t)u * 256 * VN, bv[h][u]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double g = gamma[P + h];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (valid[u]) {
+#pragma unroll
+          for (int q = 0; q < VN; ++q) acc[u][q] += g * bv[h][u][q];
+        }
+    }
+  }
+  for (; P < pe; ++P) {
     const double g = gamma[P];
     const TB* row = B + P * rowlen;
 #pragma unroll
@@ -335,7 +357,27 @@ __device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, s
   for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int q = 0; q < VN; ++q) acc[u][q] = 0.0;
-  for (size_t P = pb; P < pe; ++P) {
+  size_t P = pb;  // FP64: two aux rows per iteration, as in k_coulomb_body
+  if constexpr (sizeof(TB) == 8)
+  for (; P + 1 < pe; P += 2) {
+    double bv[2][U][VN];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (valid[u]) V::load(B + (P + h) * rowlen + e[u], bv[h][u]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double g = gamma[P + h];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (valid[u]) {
+#pragma unroll
+          for (int q = 0; q < VN; ++q) acc[u][q] += g * bv[h][u][q];
+        }
+    }
+  }
+  for (; P < pe; ++P) {
     const double g = gamma[P];
     const TB* row = B + P * rowlen;
 #pragma unroll
@@ -962,11 +1004,12 @@ int coulomb_stream(Context& c, const Tensor& t, const double* gamma, double* jpa
   const TB* B = tensor_ptr<TB>(t);
   RR_REQUIRE(B, RR_ERR_STATE, "tensor slot has no copy in the requested precision");
   const size_t rowlen = t.n * t.ld;
-  constexpr int U = 4;
+  constexpr int U = 4;  // (8 vectors per thread measured slower for FP64)
   const size_t vn = 16 / sizeof(TB);
   const size_t nvec = tri ? tri_vectors(t.n, t.ld, vn) : 0;
   const size_t per_wg = 256 * vn * U;
   const size_t nx = tri ? (nvec + 256 * U - 1) / (256 * U) : (rowlen + per_wg - 1) / per_wg;
+  // split count: measured flat between 6 and 32 splits for FP64 (one or two rounds of workgroups)
   size_t nsplit = std::max<size_t>(1, std::min<size_t>(32, (2048 + nx - 1) / nx));
   nsplit = std::min(nsplit, std::max<size_t>(1, t.naux));
   const size_t pps = (t.naux + nsplit - 1) / nsplit;
