@@ -246,7 +246,7 @@ __global__ void k_gamma_reduce(const double* gpart, int nsplit, size_t naux, dou
 }
 
 // jpart[split][e] = sum_{P in split range} gamma[P] * Bt[P][e]; each thread owns U 16-byte vectors.
-template <typename TB, int U>
+template <typename TB, int U, int PU = (sizeof(TB) == 8 ? 2 : 1)>
 __device__ __forceinline__ void k_coulomb_body(const TB* B, size_t rowlen, size_t naux, const double* gamma,
                                                  size_t p_per_split, double* jpart, unsigned bx_, unsigned by_, unsigned gx_) {
   using V = Vec16<TB>;
@@ -265,7 +265,7 @@ __device__ __forceinline__ void k_coulomb_body(const TB* B, size_t rowlen, size_
   // FP64: two aux rows per iteration, both rows' loads issued before either row is accumulated (the
   // sum still runs over P in order); narrower types keep one row (the wider rows cost occupancy)
   size_t P = pb;
-  if constexpr (sizeof(TB) == 8)
+  if constexpr (PU == 2)
   for (; P + 1 < pe; P += 2) {
     double bv[2][U][VN];
 #pragma unroll
@@ -335,7 +335,7 @@ inline size_t tri_vectors(size_t n, size_t ld, size_t VN) {
 
 // k_coulomb over the upper triangle only (symmetric tensors): jpart[split][e] for the triangle
 // elements e; the rest of jpart is left unwritten.
-template <typename TB, int U>
+template <typename TB, int U, int PU = (sizeof(TB) == 8 ? 2 : 1)>
 __device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, size_t ld, size_t nvec, size_t naux,
                                                      const double* gamma, size_t p_per_split, double* jpart, unsigned bx_, unsigned by_, unsigned gx_) {
   using V = Vec16<TB>;
@@ -358,7 +358,7 @@ __device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, s
 #pragma unroll
     for (int q = 0; q < VN; ++q) acc[u][q] = 0.0;
   size_t P = pb;  // FP64: two aux rows per iteration, as in k_coulomb_body
-  if constexpr (sizeof(TB) == 8)
+  if constexpr (PU == 2)
   for (; P + 1 < pe; P += 2) {
     double bv[2][U][VN];
 #pragma unroll
