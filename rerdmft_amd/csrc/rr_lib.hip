@@ -826,6 +826,23 @@ __global__ void k_pack_op(const double* in, size_t ld_in, char op, int cplx, siz
 void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, size_t k, const double* A, size_t lda,
                  const double* B, size_t ldb, double* C, size_t ldc, double alpha, double beta) {
   if (m == 0 || n == 0) return;
+  if (cplx && k > 0) {
+    // Library zgemm on the column-major views (a row-major X is X^T column-major): C^T = op(B)^T op(A)^T,
+    // where op(X)^T of the view is the view itself for 'N', its transpose for 'T' and its conjugate
+    // transpose for 'C'. No packing or split-K partials (the packed path costs about twice as much
+    // on the n <= 1000 products of the occupied-set recovery).
+    auto rop = [](char o) {
+      return o == 'N' ? rocblas_operation_none : (o == 'T' ? rocblas_operation_transpose : rocblas_operation_conjugate_transpose);
+    };
+    const rocblas_double_complex al = {alpha, 0.0}, be = {beta, 0.0};
+    const rocblas_status st = rocblas_zgemm(
+        c.blas, rop(opb), rop(opa), (rocblas_int)n, (rocblas_int)m, (rocblas_int)k, &al,
+        reinterpret_cast<const rocblas_double_complex*>(B), (rocblas_int)ldb,
+        reinterpret_cast<const rocblas_double_complex*>(A), (rocblas_int)lda, &be,
+        reinterpret_cast<rocblas_double_complex*>(C), (rocblas_int)ldc);
+    RR_REQUIRE(st == rocblas_status_success, RR_ERR_SOLVER, "rocblas zgemm failed");
+    return;
+  }
   const int cw = cplx ? 2 : 1;
   const size_t kp = roundup(std::max<size_t>(k, 1), 8);
   double* Ap = static_cast<double*>(wsbuf(c, WS_TMP8, m * kp * cw * sizeof(double)));

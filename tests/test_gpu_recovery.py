@@ -133,3 +133,34 @@ def test_rkb_projector_matches_eigensolver_and_oracle(ctx, oracle, method):
 def test_recovery_method_rejects_unknown_value(ctx):
     with pytest.raises(rr.GpuError):
         ctx.recovery_method(2)
+
+
+@pytest.mark.parametrize("nocc", [170, 230])
+def test_nonrel_large_occupied_set(ctx, oracle, method, nocc):
+    n, naux = 240, 40
+    weights = [1.0] * (nocc - 10) + [0.9, 0.8, 0.7, 0.6, 0.55, 0.95, 0.85, 0.75, 0.65, 0.52]
+    b, s, x, p, h = _nonrel(n, naux, weights, 671)
+    ctx.upload(SLOT, b)
+    fr = oracle.nonrel_fock_ri(b.reshape(naux, -1), x, s, h, p)
+    f0, f1, used = _run_both(ctx, lambda: ctx.nonrel_fock_ri(SLOT, h, x, s, p))
+    assert used == (1, 0)
+    assert rel(f0, fr) < 1e-11 and rel(f1, fr) < 1e-11
+
+
+@pytest.mark.parametrize("nocc", [190, 215])
+def test_x2c_large_occupied_set(ctx, oracle, method, nocc):
+    nl, naux = 110, 30
+    b = syn.ri_tensor(naux, nl, 0.5, nl / 4.0, 681)
+    s1 = syn.spd_overlap(nl, 682)
+    s = np.kron(np.eye(2), s1).astype(np.complex128)
+    x = np.kron(np.eye(2), syn.inverse_sqrt(s1)).astype(np.complex128)
+    q = syn.unitary(2 * nl, 683)[:, :nocc]
+    w = np.linspace(1.0, 0.55, nocc)
+    c = x @ q
+    p = (c * w) @ c.conj().T
+    h = syn.hermitian(2 * nl, 684)
+    ctx.upload(SLOT, b)
+    fr = oracle.x2c_fock_ri(b.reshape(naux, -1), x, s, h, p)
+    f0, f1, used = _run_both(ctx, lambda: ctx.x2c_fock_ri(SLOT, h, x, s, p))
+    assert used == (1, 0)
+    assert rel(f0, fr) < 1e-11 and rel(f1, fr) < 1e-11
