@@ -1565,13 +1565,16 @@ void solve_right_upper(Context& c, bool cplx, size_t rows, size_t r, const doubl
 // Factor of the occupied part of a Hermitian M (nx x nx) without an eigensolver:
 //   Pi = step(M - 1/2) from the Newton-Schulz sign iteration X <- X (3 - X^2) / 2 started at
 //   (M - I/2) / s, s >= ||M - I/2||_2 (s = ||(M - I/2)^8||_F^(1/8)); r = tr Pi;
-//   Q = orthonormal basis of range(Pi Omega) for a fixed pseudo-random nx x r Omega (Cholesky QR twice);
+//   Q = orthonormal basis of range(Pi Omega) for a fixed pseudo-random nx x r Omega (Cholesky QR, then
+//   Newton-Schulz orthogonalization steps);
 //   weighted: F = Q R_H^H with Q^H M Q = R_H^H R_H, so F F^H = Pi M Pi = sum_{lambda > 1/2} lambda v v^H;
 //   otherwise F = Q, F F^H = Pi = sum_{lambda > 1/2} v v^H.
 // Any F with the same F F^H gives the same exchange, so F replaces the reference's sqrt(occ)-scaled
 // eigenvectors. Writes F^T (r x nx) to rows_out and returns r, or returns -1 (no result) when the
 // iteration does not settle in a fixed number of steps (an eigenvalue at or near 1/2), when tr Pi is
 // not an integer or a Cholesky factorization fails; the caller then uses the eigensolver.
+__global__ void k_axpby(size_t count, double a, const double* x, double b, const double* y, double* out);
+
 long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool weighted, double* rows_out) {
   const int cw = cplx ? 2 : 1;
   const size_t nn = nx * nx * cw;
@@ -1627,10 +1630,20 @@ long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool we
   RR_HIP(hipGetLastError());
   dgemm_small(c, cplx, 'N', 'N', nx, r, nx, X, nx, om, r, Y, r);
   const char ct = cplx ? 'C' : 'T';
-  for (int pass = 0; pass < 2; ++pass) {
-    dgemm_small(c, cplx, ct, 'N', r, r, nx, Y, r, Y, r, G, r);  // Y^H Y
-    chol_upper(c, cplx, r, G, info + pass);
-    solve_right_upper(c, cplx, nx, r, G, Y);
+  dgemm_small(c, cplx, ct, 'N', r, r, nx, Y, r, Y, r, G, r);  // Y^H Y
+  chol_upper(c, cplx, r, G, info);
+  solve_right_upper(c, cplx, nx, r, G, Y);
+  // Cholesky QR leaves ||Q^H Q - I|| at about cond(Y)^2 eps; Newton-Schulz steps Q <- Q (3 I - Q^H Q) / 2
+  // square that error (in place of a second Cholesky QR pass: two small products per step instead of
+  // a product, a factorization and a triangular solve).
+  for (int it = 0;; ++it) {
+    dgemm_small(c, cplx, ct, 'N', r, r, nx, Y, r, Y, r, G, r);  // S = Q^H Q
+    const double e = std::sqrt(fro2(c, G, r, cw, true));          // ||S - I||_F
+    if (!(e < 0.1) || it == 4) return -1;  // also a failed factorization (NaN)
+    dgemm_small(c, cplx, 'N', 'N', nx, r, r, Y, r, G, r, om, r);  // Q S
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(nx * r * cw)), dim3(256), 0, c.stream, nx * r * cw, 1.5, Y, -0.5, om, Y);
+    RR_HIP(hipGetLastError());
+    if (e < 1e-9) break;  // the step just taken leaves about e^2 < 1e-18: rounding level
   }
   double* F = Y;
   if (weighted) {
