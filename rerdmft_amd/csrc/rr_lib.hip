@@ -1619,7 +1619,7 @@ long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool we
   if (!(std::fabs(trace - rr_) < 1e-6) || rr_ < 0.0 || rr_ > (double)nx) return -1;
   const size_t r = (size_t)rr_;
   if (r == 0) return 0;
-  // basis of range(Pi): Y = Pi Omega, two Cholesky-QR passes
+  // basis of range(Pi): Y = Pi Omega, one Cholesky-QR pass and Newton-Schulz steps
   double* om = static_cast<double*>(wsbuf(c, WS_T4 + 804, nx * r * cw * sizeof(double)));
   double* Y = static_cast<double*>(wsbuf(c, WS_T4 + 805, nx * r * cw * sizeof(double)));
   double* G = static_cast<double*>(wsbuf(c, WS_T4 + 806, r * r * cw * sizeof(double)));
