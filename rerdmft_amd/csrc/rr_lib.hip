@@ -1186,15 +1186,25 @@ __global__ __launch_bounds__(256) void k_gamma_w_blocks(const TW* W, const TW* W
 }
 
 
+// gamma[P] = sum over splits z and column groups q of part[z][P][q]: one wave per aux row P (lane l
+// takes the pairs l, l + 64, ... of the z-major list, then a fixed butterfly), workgroup bx_ holding
+// rows bx_ * waves .. (grid-stride over gx_ workgroups).
 __device__ __forceinline__ void k_gamma_w_reduce_body(const double* part, int nsplit, size_t kdim, size_t rp, size_t naux, double* gamma, unsigned bx_, unsigned by_, unsigned gx_) {
-  const size_t gpp = rp / 8, ng = kdim / 8;
-  for (size_t P = bx_ * (size_t)blockDim.x + threadIdx.x; P < naux; P += (size_t)gx_ * blockDim.x) {
+  const unsigned gpp = (unsigned)(rp / 8), cnt = (unsigned)nsplit * gpp;
+  const size_t ng = kdim / 8;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  for (size_t P = (size_t)bx_ * wpb + wave; P < naux; P += (size_t)gx_ * wpb) {
     double s = 0.0;
-    for (int z = 0; z < nsplit; ++z)
-      for (size_t q = 0; q < gpp; ++q) s += part[z * ng + P * gpp + q];
-    gamma[P] = s;
+    for (unsigned idx = lane; idx < cnt; idx += 64) {
+      const unsigned z = idx / gpp, q = idx - z * gpp;
+      s += part[z * ng + P * gpp + q];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) gamma[P] = s;
   }
 }
+// workgroups of 256 (four aux rows each) for k_gamma_w_reduce
+inline unsigned gamma_reduce_grid(size_t naux) { return (unsigned)std::max<size_t>(1, std::min<size_t>((naux + 3) / 4, 1u << 20)); }
 __global__ void k_gamma_w_reduce(const double* part, int nsplit, size_t kdim, size_t rp, size_t naux, double* gamma) {
   k_gamma_w_reduce_body(part, nsplit, kdim, rp, naux, gamma, blockIdx.x, blockIdx.y, gridDim.x);
 }
@@ -1212,7 +1222,7 @@ void gamma_from_w(Context& c, const TW* W, const TW* W2, size_t kdim, size_t rp,
     double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * ng * sizeof(double)));
     hipLaunchKernelGGL((k_gamma_w_partial<TW>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W, W2, kdim,
                        rp, n, dfac, lddf, rows, part);
-    hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux, gamma);
+    hipLaunchKernelGGL(k_gamma_w_reduce, dim3(gamma_reduce_grid(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux, gamma);
     RR_HIP(hipGetLastError());
     return;
   }
@@ -1224,7 +1234,7 @@ void gamma_from_w(Context& c, const TW* W, const TW* W2, size_t kdim, size_t rp,
   double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * (kdim / 8) * sizeof(double)));
   hipLaunchKernelGGL((k_gamma_w_blocks<TW, NPP>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W, W2,
                      kdim, rp, n, naux, dfac, lddf, rows, part);
-  hipLaunchKernelGGL(k_gamma_w_reduce, dim3(grid_for(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux, gamma);
+  hipLaunchKernelGGL(k_gamma_w_reduce, dim3(gamma_reduce_grid(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux, gamma);
   RR_HIP(hipGetLastError());
 }
 
@@ -2994,7 +3004,7 @@ BatchPlanT<TI>* build_plan(Context& c, const std::vector<Tensor>& ts, int prec, 
         rows = std::max<size_t>(8, (n * gx + 4095) / 4096);
         gsplit = (int)((n + rows - 1) / rows);
         p->gam.add({W, kdim, rp, n, naux, dfac, rp, rows, gpart, (unsigned)gx}, gx * (size_t)gsplit);
-        const unsigned gr = grid_for(naux);
+        const unsigned gr = gamma_reduce_grid(naux);
         p->gam_red.add({gpart, gsplit, kdim, rp, naux, gamma, gr}, gr);
       }
       int ksplit;
