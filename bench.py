@@ -50,6 +50,9 @@ def parse():
                    help="trial densities in the shared-tensor leg (0: skip); n = 800 (100 occ + 700 virt), n_aux = 3200")
     p.add_argument("--relativistic", type=int, default=1,
                    help="1: time the X2C (n=400, n_aux=2400) and 4c RKB (nL=300, nS=700, n_aux=4200) builds")
+    p.add_argument("--rows", type=int, default=1,
+                   help="1: time the other entry points (RKB from a density, Cholesky builds, MO transforms, "
+                        "functional-stage products, metric) at mid-size shapes")
     p.add_argument("--copy-probe-gib", type=float, default=0.0,
                    help="only run a device copy of this many GiB (calibrates HBM byte counters) and exit")
     return p.parse_args()
@@ -289,6 +292,7 @@ def main():
         "batch_uneven_molecules": batch_leg(torch, rr, ctx, a.batch_molecules) if a.batch_molecules > 0 else None,
         "sigma_trial_densities": sigma_leg(torch, rr, ctx, a.sigma_densities) if a.sigma_densities > 0 else None,
         "relativistic": relativistic_leg(torch, rr, ctx, cpu=a.cpu_sample > 0) if a.relativistic else None,
+        "other_rows": rows_leg(torch, rr, ctx) if a.rows else None,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -433,6 +437,128 @@ def relativistic_cpu(x2c_shape, rkb_shape):
     dt = time.perf_counter() - t0
     out["rkb"] = {"value": nsamp / (dt * naux), "unit": "Fock builds/s", "cores": orc.max_threads(), "kind": "port",
                   "sample": f"riFockTwoElectronDirect over {nsamp} of {naux} aux rows ({dt:.2f} s), scaled by {naux}/{nsamp}"}
+    return out
+
+
+def rows_leg(torch, rr, ctx, reps=3):
+    """Per-call device time of the remaining entry points at mid-size shapes: inputs and outputs are
+    device tensors (no host transfer inside the timed calls), FP64, one warm-up call each. A row whose
+    call raises is reported with its error instead of a time."""
+    from rerdmft_amd import synthetic as syn
+    dev = torch.device("cuda:0")
+    D = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    out = {}
+
+    def row(name, fn, flop=None, **info):
+        try:
+            ms = 1e3 * _timed(torch, fn, reps)
+            r = {"ms_per_call": round(ms, 3), **info}
+            if flop:
+                r["TFLOP_per_s"] = round(flop / ms / 1e9, 1)
+            out[name] = r
+        except Exception as e:  # reported, not fatal: the other rows still run
+            out[name] = {"error": f"{type(e).__name__}: {e}"[:300], **info}
+
+    # a4: RKB from a density (recovery in the RKB basis, V transforms, 4-flavor build)
+    nl, ns, naux, nocc = 150, 350, 1200, 120
+    try:
+        ctx.generate(5, naux, nl, 0.4, 20.0, SEED + 31, rr.COPY_FP64)
+        ctx.generate(6, naux, ns, 0.2, 40.0, SEED + 32, rr.COPY_FP64)
+        n_ukb, n_rkb = 2 * nl + 2 * ns, 4 * nl
+        g = syn.rng(SEED + 33)
+        v = (g.standard_normal((n_ukb, n_rkb)) + 1j * g.standard_normal((n_ukb, n_rkb))) * 0.3
+        sr = v.conj().T @ v + 0.5 * np.eye(n_rkb)
+        w, u = np.linalg.eigh(sr)
+        x = u / np.sqrt(w)[None, :]
+        c = x @ syn.unitary(n_rkb, SEED + 34)[:, :nocc]
+        pd = (c * np.linspace(1.0, 0.6, nocc)) @ c.conj().T
+        h = syn.hermitian(n_rkb, SEED + 35)
+        args = [D(a) for a in (h, v, sr, x, pd)]
+        fo = torch.empty((n_rkb, n_rkb), dtype=torch.complex128, device=dev)
+        row("a4_rkb_fock_ri", lambda: ctx.rkb_fock_ri(5, 6, nl, ns, *args, out=fo),
+            shape=f"nL={nl} nS={ns} n_aux={naux} n_occ={nocc}")
+    except Exception as e:
+        out["a4_rkb_fock_ri"] = {"error": f"{type(e).__name__}: {e}"[:300]}
+    for sl in (5, 6):
+        try:
+            ctx.release(sl)
+        except Exception:
+            pass
+
+    # a5 / a15 / a8 / a10 on one set of n x n vectors
+    n, nchol, nmo = 500, 1600, 250
+    try:
+        ctx.generate(7, nchol, n, 1.0, 50.0, SEED + 41, rr.COPY_FP64)
+        g = syn.rng(SEED + 42)
+        h = D(syn.symmetric(n, SEED + 43))
+        c0 = np.linalg.qr(g.standard_normal((n, 100)))[0]
+        pdn = D(2.0 * c0 @ c0.T)
+        fo = torch.empty((n, n), dtype=torch.float64, device=dev)
+        row("a5_nonrel_fock_cholesky", lambda: ctx.nonrel_fock_cholesky(7, h, pdn, out=fo), flop=4.0 * n ** 3 * nchol,
+            shape=f"n={n} n_chol={nchol}")
+        cx = np.linalg.qr(g.standard_normal((2 * n, 100)) + 1j * g.standard_normal((2 * n, 100)))[0]
+        hx, px = D(syn.hermitian(2 * n, SEED + 44)), D(cx @ cx.conj().T)
+        fx = torch.empty((2 * n, 2 * n), dtype=torch.complex128, device=dev)
+        row("a5_x2c_fock_cholesky", lambda: ctx.x2c_fock_cholesky(7, hx, px, out=fx), shape=f"n={n} n_chol={nchol}")
+        cm = D(np.linalg.qr(g.standard_normal((n, nmo)))[0])
+        mo = torch.empty((nchol, nmo, nmo), dtype=torch.float64, device=dev)
+        mo_flop = nchol * (2.0 * n * n * nmo + 2.0 * n * nmo * nmo)
+        row("a15_cholesky_to_mo_spatial", lambda: ctx.cholesky_to_mo_spatial(7, cm, out=mo), flop=mo_flop,
+            shape=f"n={n} n_chol={nchol} n_mo={nmo}")
+        row("a8_build_ri_nonrel_spin_mo_eri", lambda: ctx.build_ri_nonrel_spin_mo_eri(7, cm, out=mo), flop=mo_flop,
+            shape=f"n={n} n_aux={nchol} n_mo={nmo}")
+        occ = D(g.uniform(0.05, 0.95, 2 * n))
+        h2, x2 = D(g.standard_normal((2 * n, 2 * n))), D(g.standard_normal((2 * n, 2 * n)))
+        cfull = D(np.linalg.qr(g.standard_normal((n, n)))[0])
+        fm = torch.empty((2 * n, 2 * n), dtype=torch.float64, device=dev)
+        row("a10_ri_mo_fock_nonrel", lambda: ctx.ri_mo_fock_nonrel(7, h, cfull, occ, h2, x2, out=fm),
+            shape=f"n={n} n_aux={nchol} (MO tensor, h_MO and the generalized Fock)")
+        del mo
+    except Exception as e:
+        out["a5_a8_a10_a15_setup"] = {"error": f"{type(e).__name__}: {e}"[:300]}
+    try:
+        ctx.release(7)
+    except Exception:
+        pass
+
+    # a9 / a13 on a device MO tensor
+    n, naux = 200, 800
+    try:
+        g = syn.rng(SEED + 51)
+        b = torch.randn((naux, n, n), dtype=torch.float64, device=dev)
+        b = 0.5 * (b + b.transpose(1, 2))
+        hm = D(syn.symmetric(n, SEED + 52))
+        occ = D(g.uniform(0.05, 0.95, n))
+        h2 = g.standard_normal((n, n))
+        x2 = g.standard_normal((n, n))
+        h2d, x2d = D(h2), D(x2)
+        fo = torch.empty((n, n), dtype=torch.float64, device=dev)
+        row("a9_ri_hx_fock_gemm", lambda: ctx.ri_hx_fock_gemm(b, hm, occ, h2d, x2d, out=fo), shape=f"n={n} n_aux={naux}")
+        pairs = [(p_, q_) for p_ in range(n) for q_ in range(p_)]
+        vv = D(g.standard_normal(len(pairs)))
+        h2s, x2s = D(0.5 * (h2 + h2.T)), D(0.5 * (x2 + x2.T))
+        row("a13_jk_only_sigma_hessian_vector_ri", lambda: ctx.jk_only_sigma_hessian_vector_ri(hm, b, occ, h2s, x2s, pairs, vv),
+            shape=f"n={n} n_aux={naux} pairs={len(pairs)} (toolkit a11 and pair contractions a12 inside)")
+        del b
+    except Exception as e:
+        out["a9_a13_setup"] = {"error": f"{type(e).__name__}: {e}"[:300]}
+
+    # a14: metric orthogonalization and the three-centre contraction
+    n, naux_raw = 200, 2000
+    try:
+        g = syn.rng(SEED + 61)
+        q = np.linalg.qr(g.standard_normal((naux_raw, naux_raw)))[0]
+        metric = D((q * np.geomspace(1.0, 1e-6, naux_raw)) @ q.T)
+        row("a14_canonical_orthogonalize", lambda: ctx.canonical_orthogonalize(metric, 1e-10), shape=f"n_aux={naux_raw}")
+        xk = D(ctx.canonical_orthogonalize(metric, 1e-10))
+        raw = torch.randn((naux_raw, n * n), dtype=torch.float64, device=dev)
+        bo = torch.empty((xk.shape[1], n * n), dtype=torch.float64, device=dev)
+        row("a14_ri_three_center_tensor", lambda: ctx.ri_three_center_tensor(raw, xk, out=bo),
+            flop=2.0 * naux_raw * xk.shape[1] * n * n, shape=f"n={n} n_aux={naux_raw}")
+        del raw, bo
+    except Exception as e:
+        out["a14_setup"] = {"error": f"{type(e).__name__}: {e}"[:300]}
+    torch.cuda.empty_cache()
     return out
 
 

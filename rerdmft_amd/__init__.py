@@ -388,13 +388,13 @@ class Context:
     # ---- functional stage: MO RI tensors (RiGemmFock / RiSigma*) ---------------------------------
     @staticmethod
     def _elem(b):
-        return np.complex128 if np.iscomplexobj(b) else np.float64
+        return np.complex128 if _is_complex(b) else np.float64
 
     def ri_hx_fock_gemm(self, b, h_mo, occupations, two_rdm_h, two_rdm_x, pair_of=None, two_rdm_l1=None,
                         two_rdm_l2=None, out=None):
         """riHxFockGemm -- src/Hessian_opt/RiGemmFock.cpp:53-115 (hartreeExchangeFockMatrix /
         jkOnlyFockMatrix on an RI MO tensor b of shape (n_aux, n, n) or (n_aux, n*n))."""
-        cplx = np.iscomplexobj(b) or np.iscomplexobj(h_mo)
+        cplx = _is_complex(b, h_mo)
         dt = np.complex128 if cplx else np.float64
         n = h_mo.shape[0]
         naux = b.shape[0]
@@ -425,7 +425,7 @@ class Context:
     def ri_pair_contract(self, target, sign, n, naux, f, g):
         """pairContract -- src/Hessian_opt/RiSigmaGemm.cpp:76-101. f, g: dicts with optional keys
         s (n_aux, n, n), idx (n, n) uint32, w (n, n), row (n_aux, n), post (n, n). target updated in place."""
-        dt = np.complex128 if np.iscomplexobj(target) else np.float64
+        dt = np.complex128 if _is_complex(target) else np.float64
         keep = []
 
         def mk(d):
@@ -449,7 +449,7 @@ class Context:
 
     def ri_diag_contract(self, target, s, ua=None, ub=None):
         """diagContract -- src/Hessian_opt/RiSigmaGemm.cpp:103-121 (target updated in place)."""
-        dt = np.complex128 if np.iscomplexobj(target) else np.float64
+        dt = np.complex128 if _is_complex(target) else np.float64
         n, naux = target.shape[0], s.shape[0]
         ss, a, b_ = _Arr(s, dt), _Arr(ua, dt), _Arr(ub, dt)
         t = np.ascontiguousarray(target, dtype=dt)
@@ -460,7 +460,7 @@ class Context:
 
     def ri_accum_u(self, u, alpha, d, wm):
         """accumU -- src/Hessian_opt/RiSigmaGemm.cpp:123-132 (u updated in place)."""
-        dt = np.complex128 if np.iscomplexobj(u) else np.float64
+        dt = np.complex128 if _is_complex(u) else np.float64
         naux, n = u.shape
         dd, ww = _Arr(d, dt), _f64(wm)
         t = np.ascontiguousarray(u, dtype=dt)
