@@ -543,6 +543,18 @@ def rows_leg(torch, rr, ctx, reps=3):
     except Exception as e:
         out["a9_a13_setup"] = {"error": f"{type(e).__name__}: {e}"[:300]}
 
+    # a7: rotation of a complex MO tensor, B'_P = U^H B_P U
+    nt, naux = 400, 600
+    try:
+        bz = torch.randn((naux, nt, nt), dtype=torch.complex128, device=dev)
+        uz = D(syn.unitary(nt, SEED + 55))
+        bo = torch.empty_like(bz)
+        row("a7_ri_mo_eri_rotated", lambda: ctx.ri_mo_eri_rotated(bz, uz, out=bo), flop=naux * 2 * 8.0 * nt ** 3,
+            shape=f"n_tot={nt} n_aux={naux} (complex)")
+        del bz, bo
+    except Exception as e:
+        out["a7_setup"] = {"error": f"{type(e).__name__}: {e}"[:300]}
+
     # a14: metric orthogonalization and the three-centre contraction
     n, naux_raw = 200, 2000
     try:
