@@ -50,7 +50,7 @@ def parse():
                    help="trial densities in the shared-tensor leg (0: skip); n = 800 (100 occ + 700 virt), n_aux = 3200")
     p.add_argument("--relativistic", type=int, default=1,
                    help="1: time the X2C (n=400, n_aux=2400) and 4c RKB (nL=300, nS=700, n_aux=4200) builds")
-    p.add_argument("--rows", type=int, default=1,
+    p.add_argument("--rows", type=int, default=0,
                    help="1: time the other entry points (RKB from a density, Cholesky builds, MO transforms, "
                         "functional-stage products, metric) at mid-size shapes")
     p.add_argument("--copy-probe-gib", type=float, default=0.0,

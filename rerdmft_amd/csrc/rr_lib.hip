@@ -2550,23 +2550,6 @@ void cholesky_exchange(Context& c, const Tensor& t, int prec, const double* dPop
   g.M = (int)n, g.N = (int)(cplx_rows ? 2 * n : n), g.K = (int)ld, g.kbatch = (int)nchol;
   GemmSpec sp;
   sp.prec = op, sp.out = P64, sp.cfg = 2;
-  // The output alone gives few 128 x 128 tiles (16 at n = 500) for a contraction over n_chol * n:
-  // split the flat k into up to 2048 workgroups' worth of partial sums (at most 512 MB of them),
-  // summed in split order afterwards.
-  const size_t tiles = ((n + 127) / 128) * ((g.N + 127) / 128);
-  const size_t ktot = nchol * ld, slab = prec == P64 ? 16 : 32, outsz = n * (size_t)g.N;
-  size_t ns = std::min<size_t>({256, std::max<size_t>(1, 2048 / tiles), std::max<size_t>(1, ktot / 4096),
-                                std::max<size_t>(1, (size_t(512) << 20) / (outsz * sizeof(double)))});
-  if (ns > 1 && ldk == (size_t)g.N) {
-    const size_t kper = roundup((ktot + ns - 1) / ns, slab);
-    const int nsplit = (int)((ktot + kper - 1) / kper);
-    double* part = static_cast<double*>(wsbuf(c, WS_KPART, (size_t)nsplit * outsz * sizeof(double)));
-    g.C = part, g.sCz = outsz, g.kper = (int)kper, g.nsplit = nsplit;
-    gemm_auto(c, sp, g);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(grid_for(outsz)), dim3(256), 0, c.stream, part, nsplit, outsz, outsz, kout);
-    RR_HIP(hipGetLastError());
-    return;
-  }
   gemm_auto(c, sp, g);
 }
 
