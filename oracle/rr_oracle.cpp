@@ -535,13 +535,15 @@ int oracle_rkb_fock_ri(std::size_t nl, std::size_t ns, std::size_t naux, const d
     const std::vector<C> c_rkb = recoverComplexOcc(n_rkb, nx, s, dens, x, occ);
     const std::size_t n_occ = occ.size();
     const std::vector<C> c_ukb = zmul(n_ukb, n_rkb, n_occ, v, c_rkb.data());
-    const std::vector<C> p_ukb = zmul(n_ukb, n_rkb, n_ukb, v, zmul(n_rkb, n_rkb, n_ukb, dens, v, 'N', 'C').data());
+    const std::vector<C> p_ukb =
+        zmul(n_ukb, n_rkb, n_ukb, v, zmul(n_rkb, n_rkb, n_ukb, dens, v, 'N', planted("rkb_pvt") ? 'T' : 'C').data());
     std::vector<C> g(n_ukb * n_ukb);
     int rc = oracle_ri_fock_two_electron_direct(nl, ns, naux, eri3_ll, eri3_ss, reinterpret_cast<const double*>(p_ukb.data()),
                                                 reinterpret_cast<const double*>(c_ukb.data()), n_occ, occ.data(),
                                                 reinterpret_cast<double*>(g.data()));
     if (rc) return rc;
-    const std::vector<C> vgv = zmul(n_rkb, n_ukb, n_rkb, v, zmul(n_ukb, n_ukb, n_rkb, g.data(), v).data(), 'C', 'N');
+    const std::vector<C> vgv =
+        zmul(n_rkb, n_ukb, n_rkb, v, zmul(n_ukb, n_ukb, n_rkb, g.data(), v).data(), planted("rkb_vt") ? 'T' : 'C', 'N');
     for (std::size_t i = 0; i < n_rkb * n_rkb; ++i) fock[i] = h[i] + vgv[i];
     return 0;
   } catch (const std::exception& e) {

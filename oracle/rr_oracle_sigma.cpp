@@ -355,7 +355,7 @@ struct Sigma {
     std::vector<T> Uh = U();
     accumU(Uh, 1.0, n, naux, t.bdiag.data(), hc);
     const std::vector<T> nUh = neg(Uh);
-    pc(Phi, 1.0, op(Bm, tx()), op(K1, xt(), [&](Z u, Z b) { return HC(b, u); }));
+    pc(Phi, 1.0, op(Bm, tx()), op(K1, xt(), [&](Z u, Z b) { return plantedSigma("hx_h_as_x") ? XC(b, u) : HC(b, u); }));
     diagContract(Phi, n, naux, K1, static_cast<const T*>(nullptr), nUh.data());
     pc(Phi, 1.0, op(K2, tx(), [&](Z u, Z a) { return HC(a, u); }), op(Bm, xt()));
     diagContract(Phi, n, naux, K2, nUh.data(), static_cast<const T*>(nullptr));
@@ -413,7 +413,8 @@ struct Sigma {
     const T* K2 = t.kb2.data();
     const Z nn = n;
     pc(Phi, -1.0, op(Bm, tx()),
-       op(K2, [&](Z u, Z b) { return pr(u) * nn + pr(b); }, [&](Z u, Z b) { return 2.0 * (L1(u, pr(b)) + L2(u, b)); }));
+       op(K2, [&](Z u, Z b) { return pr(u) * nn + pr(b); },
+          [&](Z u, Z b) { return 2.0 * (L1(u, pr(b)) + (plantedSigma("hx_l2_flip") ? L2(b, u) : L2(u, b))); }));
     pc(Phi, 1.0, op(Bm, [&](Z r, Z a) { return pr(r) * nn + a; }),
        op(K1, [&](Z r, Z b) { return r * nn + pr(b); }, [&](Z r, Z b) { return 2.0 * (L1(r, b) + L2(r, pr(b))); }));
     pc(Phi, -1.0, op(K1, tx()),

@@ -257,6 +257,52 @@ def test_rkb_fock_recovery_and_embedding(oracle):
     assert np.max(np.abs(f - (h + g))) < 1e-10
 
 
+def rkb_rect_case(nl=3, ns=4, naux=7, n_rkb=10, nocc=3, seed=41):
+    """A rectangular complex RKB→UKB map V (n_ukb × n_rkb, n_ukb = 2nl + 2ns = 14) and an idempotent
+    RKB density P = C C^H with C^H S C = I."""
+    bll, bss, _, _, _ = rkb_case(nl, ns, naux, nocc, seed)
+    n_ukb = 2 * nl + 2 * ns
+    g = syn.rng(seed + 10)
+    v = g.standard_normal((n_ukb, n_rkb)) + 1j * g.standard_normal((n_ukb, n_rkb))
+    a = g.standard_normal((n_rkb, n_rkb)) + 1j * g.standard_normal((n_rkb, n_rkb))
+    s = a @ a.conj().T / n_rkb + np.eye(n_rkb)
+    w, u = np.linalg.eigh(s)
+    x = (u / np.sqrt(w)) @ u.conj().T
+    c = x @ syn.unitary(n_rkb, seed + 11)[:, :nocc]
+    h = syn.hermitian(n_rkb, seed + 12)
+    return bll, bss, v, s, x, c, c @ c.conj().T, h
+
+
+def dense_ukb_g(bll, bss, nl, ns, p_ukb, d_occ):
+    """G(A,R) = sum P(S,Q) <AQ|RS> - sum D(S,Q) <AQ|SR> on the dense four-flavor tensor."""
+    eri = ukb_dense(bll, bss, nl, ns)
+    return np.einsum("sq,aqrs->ar", p_ukb.real, eri) - np.einsum("sq,aqsr->ar", d_occ, eri)
+
+
+def test_rkb_fock_rectangular_v_matches_dense_embedding(oracle):
+    # F = h + V^H G[V P V^H] V with G from the dense UKB tensor and the occupied set V C (an
+    # idempotent RKB density recovers C up to a unitary, which K does not see).
+    nl, ns, naux = 3, 4, 7
+    bll, bss, v, s, x, c, p, h = rkb_rect_case(nl, ns, naux)
+    f = oracle.rkb_fock_ri(nl, ns, bll.reshape(naux, -1), bss.reshape(naux, -1), v, s, x, h, p)
+    p_ukb = v @ p @ v.conj().T
+    g = dense_ukb_g(bll, bss, nl, ns, p_ukb, (v @ c) @ (v @ c).conj().T)
+    ref = h + v.conj().T @ g @ v
+    assert np.max(np.abs(f - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+    assert np.max(np.abs(f - f.conj().T)) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("plant", ["rkb_vt", "rkb_pvt"])
+def test_rkb_fock_planted_dagger_is_rejected(oracle, monkeypatch, plant):
+    nl, ns, naux = 3, 4, 7
+    bll, bss, v, s, x, c, p, h = rkb_rect_case(nl, ns, naux)
+    p_ukb = v @ p @ v.conj().T
+    ref = h + v.conj().T @ dense_ukb_g(bll, bss, nl, ns, p_ukb, (v @ c) @ (v @ c).conj().T) @ v
+    monkeypatch.setenv("RR_ORACLE_PLANT", plant)
+    f = oracle.rkb_fock_ri(nl, ns, bll.reshape(naux, -1), bss.reshape(naux, -1), v, s, x, h, p)
+    assert np.max(np.abs(f - ref)) > 1e-3 * max(1.0, np.max(np.abs(ref)))
+
+
 # ---------------------------------------------------------------------------------------------
 # MO-basis tensors (RiMoEri.cpp:20-121, NonRelSpinRiMoEri.cpp:13-95)
 # ---------------------------------------------------------------------------------------------
