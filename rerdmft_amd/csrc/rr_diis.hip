@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <deque>
+#include <limits>
 
 #include "rr_core.hpp"
 
@@ -68,34 +69,53 @@ __global__ void k_diis_sub(const double* a, const double* b, size_t count, doubl
     out[k] = a[k] - b[k];
 }
 
-// Gaussian elimination with partial pivoting on the dense n x n system; false when
-// min|pivot| / max|pivot| is not above `tolerance` (the reference's singularity test, DIIS.cpp:18-45).
-bool solve_dense(std::vector<double> m, std::vector<double> rhs, size_t n, double tolerance, std::vector<double>& x) {
-  std::vector<double> piv(n);
-  for (size_t k = 0; k < n; ++k) {
-    size_t best = k;
-    for (size_t i = k + 1; i < n; ++i)
-      if (std::fabs(m[i * n + k]) > std::fabs(m[best * n + k])) best = i;
-    if (best != k) {
-      for (size_t j = 0; j < n; ++j) std::swap(m[k * n + j], m[best * n + j]);
-      std::swap(rhs[k], rhs[best]);
+// Solves the (m+1) x (m+1) Pulay system by an LU factorisation with row pivoting (P A = L U, the unit
+// lower factor kept in place of the eliminated entries, rows addressed through a permutation table),
+// then forward and back substitution. The system is refused, as in the reference (DIIS.cpp:18-45),
+// when a pivot is exactly zero or when min_k |U_kk| is not above tolerance * max_k |U_kk|; the pivot
+// of a column is the first remaining row of largest magnitude, so the U diagonal, and with it that
+// decision, is the one partial pivoting defines.
+bool lu_solve_pivot_ratio(std::vector<double> a, const std::vector<double>& b, size_t n, double tolerance,
+                          std::vector<double>& x) {
+  std::vector<size_t> row(n);
+  for (size_t i = 0; i < n; ++i) row[i] = i;
+  double umax = 0.0, umin = std::numeric_limits<double>::infinity();
+  for (size_t c = 0; c < n; ++c) {
+    size_t p = c;
+    double big = std::fabs(a[row[c] * n + c]);
+    for (size_t r = c + 1; r < n; ++r) {
+      const double mag = std::fabs(a[row[r] * n + c]);
+      if (mag > big) {
+        big = mag;
+        p = r;
+      }
     }
-    piv[k] = std::fabs(m[k * n + k]);
-    if (piv[k] == 0.0) return false;
-    for (size_t i = k + 1; i < n; ++i) {
-      const double f = m[i * n + k] / m[k * n + k];
-      for (size_t j = k; j < n; ++j) m[i * n + j] -= f * m[k * n + j];
-      rhs[i] -= f * rhs[k];
+    std::swap(row[c], row[p]);
+    if (big == 0.0) return false;
+    umax = std::max(umax, big);
+    umin = std::min(umin, big);
+    const double* urow = a.data() + row[c] * n;
+    for (size_t r = c + 1; r < n; ++r) {
+      double* lrow = a.data() + row[r] * n;
+      const double l = lrow[c] / urow[c];
+      lrow[c] = l;
+      for (size_t j = c + 1; j < n; ++j) lrow[j] -= l * urow[j];
     }
   }
-  const double pmax = *std::max_element(piv.begin(), piv.end());
-  const double pmin = *std::min_element(piv.begin(), piv.end());
-  if (!(pmin > tolerance * pmax)) return false;
+  if (!(umin > tolerance * umax)) return false;
+  std::vector<double> y(n);
+  for (size_t i = 0; i < n; ++i) {
+    const double* lrow = a.data() + row[i] * n;
+    double s = b[row[i]];
+    for (size_t k = 0; k < i; ++k) s -= lrow[k] * y[k];
+    y[i] = s;
+  }
   x.assign(n, 0.0);
   for (size_t i = n; i-- > 0;) {
-    double s = rhs[i];
-    for (size_t j = i + 1; j < n; ++j) s -= m[i * n + j] * x[j];
-    x[i] = s / m[i * n + i];
+    const double* urow = a.data() + row[i] * n;
+    double s = y[i];
+    for (size_t j = i + 1; j < n; ++j) s -= urow[j] * x[j];
+    x[i] = s / urow[i];
   }
   return true;
 }
@@ -236,7 +256,7 @@ int rr_diis_extrapolate(rr_handle h, rr_diis_t d, size_t n_error, const double* 
         for (size_t j = 0; j < m; ++j) a[i * n + j] = d->b[i][j] / scale;
       for (size_t i = 0; i < m; ++i) a[i * n + m] = a[m * n + i] = -1.0;
       rhs[m] = -1.0;
-      if (!solve_dense(std::move(a), std::move(rhs), n, 1e-14, x)) continue;
+      if (!lu_solve_pivot_ratio(std::move(a), std::move(rhs), n, 1e-14, x)) continue;
       Combo cb;
       cb.n = (int)m;
       for (size_t i = 0; i < m; ++i) cb.v[i] = d->values[i], cb.w[i] = x[i];
