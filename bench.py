@@ -509,7 +509,9 @@ def rows_leg(torch, rr, ctx, reps=3):
             shape=f"n={n} n_aux={nchol} n_mo={nmo}")
         occ = D(g.uniform(0.05, 0.95, 2 * n))
         h2, x2 = D(g.standard_normal((2 * n, 2 * n))), D(g.standard_normal((2 * n, 2 * n)))
-        cfull = D(np.linalg.qr(g.standard_normal((n, n)))[0])
+        # spin-orbital coefficients, n x 2n (RiNonRelFullOptFock.cpp:53-98): alpha and beta columns
+        q = np.linalg.qr(g.standard_normal((n, n)))[0]
+        cfull = D(np.concatenate([q, q], axis=1))
         fm = torch.empty((2 * n, 2 * n), dtype=torch.float64, device=dev)
         row("a10_ri_mo_fock_nonrel", lambda: ctx.ri_mo_fock_nonrel(7, h, cfull, occ, h2, x2, out=fm),
             shape=f"n={n} n_aux={nchol} (MO tensor, h_MO and the generalized Fock)")

@@ -76,11 +76,26 @@ def lib():
 _sz = ctypes.c_size_t
 
 
-class _Arr:
-    """Pointer view of a numpy array or torch tensor (keeps a contiguous copy alive)."""
+def _count(x):
+    """Element count of a numpy array or torch tensor (complex values count once)."""
+    return int(x.numel()) if type(x).__module__.startswith("torch") else int(np.size(x))
 
-    def __init__(self, x, dtype):
+
+def _need(x, count, what):
+    """The C ABI reads or writes exactly `count` elements through a plain pointer; an array of any other
+    size would be read or written past its end, so the size is checked here, before the call."""
+    if x is not None and _count(x) != count:
+        raise ValueError(f"{what}: expected {count} elements, got {_count(x)} (shape {tuple(x.shape)})")
+
+
+class _Arr:
+    """Pointer view of a numpy array or torch tensor (keeps a contiguous copy alive). With `count`,
+    the array must hold exactly that many elements."""
+
+    def __init__(self, x, dtype, count=None, what="array"):
         self.torch = False
+        if count is not None:
+            _need(x, count, what)
         if x is None:
             self.ptr = ctypes.c_void_p(0)
             self.obj = None
@@ -100,8 +115,8 @@ class _Arr:
             self.ptr = ctypes.c_void_p(a.ctypes.data)
 
 
-def _f64(x):
-    return _Arr(x, np.float64)
+def _f64(x, count=None, what="array"):
+    return _Arr(x, np.float64, count, what)
 
 
 def _is_complex(*xs):
@@ -114,13 +129,13 @@ def _is_complex(*xs):
     return False
 
 
-def _c128(x):
-    return _Arr(x, np.complex128)
+def _c128(x, count=None, what="array"):
+    return _Arr(x, np.complex128, count, what)
 
 
 def _out(out, shape, dtype):
     if out is not None:
-        return out, _Arr(out, dtype).ptr
+        return out, _Arr(out, dtype, int(np.prod(shape)), "out").ptr
     a = np.empty(shape, dtype=dtype)
     return a, ctypes.c_void_p(a.ctypes.data)
 
@@ -209,7 +224,9 @@ class Context:
     def nonrel_fock_ri(self, slot, h_core, x_large, overlap, density, precision=FP64, return_jk=False, out=None):
         """nonRelFockMatrix(h, RiNonRelEriSource, P) -- src/NON_REL/NonRelFockMatrixRi.cpp:27-100."""
         n = h_core.shape[0]
-        h, x, s, d = _f64(h_core), _f64(x_large), _f64(overlap), _f64(density)
+        nx = x_large.shape[1]
+        h, x = _f64(h_core, n * n, "h_core"), _f64(x_large, n * nx, "x_large")
+        s, d = _f64(overlap, n * n, "overlap"), _f64(density, n * n, "density")
         f, fp = _out(out, (n, n), np.float64)
         j = k = None
         jp = kp = ctypes.c_void_p(0)
@@ -226,7 +243,7 @@ class Context:
         n = lfac.shape[0] if lfac is not None else density.shape[0]
         # density=None: J of D = L R^T from the half-transformed factors (fused Coulomb contraction)
         r = lfac.shape[1] if lfac is not None else 0
-        d, l, rr_ = _f64(density), _f64(lfac), _f64(rfac)
+        d, l, rr_ = _f64(density, n * n, "density"), _f64(lfac, n * r, "lfac"), _f64(rfac, n * r, "rfac")
         j, jp = _out(j_out, (n, n), np.float64) if want_j else (None, ctypes.c_void_p(0))
         k, kp = _out(k_out, (n, n), np.float64) if want_k else (None, ctypes.c_void_p(0))
         self._call("rr_ri_jk_factor", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), d.ptr, l.ptr, rr_.ptr,
@@ -238,7 +255,7 @@ class Context:
         Returns (J (n_dens, n, n) or None, K or None, replayed)."""
         n, r = lfac.shape
         nd = rfacs.shape[0]
-        l, rf = _f64(lfac), _f64(rfacs)
+        l, rf = _f64(lfac), _f64(rfacs, nd * n * r, "rfacs")
         j, jp = _out(j_out, (nd, n, n), np.float64) if want_j else (None, ctypes.c_void_p(0))
         k, kp = _out(k_out, (nd, n, n), np.float64) if want_k else (None, ctypes.c_void_p(0))
         rep = ctypes.c_int(0)
@@ -249,7 +266,7 @@ class Context:
     def ri_jk_delta(self, slot, c_new, c_old, precision=FP64, dj_out=None, dk_out=None):
         """dJ, dK of D_new - D_old with D = 2 C C^T (aligned-difference factorization)."""
         n, no = c_new.shape
-        a, b = _f64(c_new), _f64(c_old)
+        a, b = _f64(c_new), _f64(c_old, n * no, "c_old")
         dj, djp = _out(dj_out, (n, n), np.float64)
         dk, dkp = _out(dk_out, (n, n), np.float64)
         self._call("rr_ri_jk_delta", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), _sz(no), a.ptr, b.ptr, djp,
@@ -260,7 +277,9 @@ class Context:
     def x2c_fock_ri(self, slot, h_x2c, x_large, overlap, density, precision=FP64, out=None):
         """x2cFockMatrix(h, RiX2cEriSource, P) -- src/X2C_DHF/X2C_FockMatrixRi.cpp:35-162."""
         n2 = h_x2c.shape[0]
-        h, x, s, d = _c128(h_x2c), _c128(x_large), _c128(overlap), _c128(density)
+        nx = x_large.shape[1]
+        h, x = _c128(h_x2c, n2 * n2, "h_x2c"), _c128(x_large, n2 * nx, "x_large")
+        s, d = _c128(overlap, n2 * n2, "overlap"), _c128(density, n2 * n2, "density")
         f, fp = _out(out, (n2, n2), np.complex128)
         self._call("rr_x2c_fock_ri", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n2 // 2), h.ptr, x.ptr,
                    _sz(x_large.shape[1]), s.ptr, d.ptr, fp)
@@ -271,7 +290,8 @@ class Context:
                                     precision=FP64, out=None):
         """riFockTwoElectronDirect -- src/UKB/UkbFockMatrixRi.cpp:35-183."""
         n = 2 * nl + 2 * ns
-        d, c = _c128(density_ukb), _c128(c_occ_ukb)
+        no = len(occupations)
+        d, c = _c128(density_ukb, n * n, "density_ukb"), _c128(c_occ_ukb, n * no, "c_occ_ukb")
         occ = _f64(occupations)
         g, gp = _out(out, (n, n), np.complex128)
         self._call("rr_ri_fock_two_electron_direct", ctypes.c_int(slot_ll), ctypes.c_int(slot_ss),
@@ -282,7 +302,10 @@ class Context:
                     out=None):
         """rkbFockMatrix(h, RiDirectEriSource, P) -- src/UKB/UkbFockMatrixRi.cpp:185-216."""
         n_rkb = h_rkb.shape[0]
-        h, v, s, x, d = map(_c128, (h_rkb, v_total, s_full, x_full, density))
+        nu, nx = 2 * nl + 2 * ns, x_full.shape[1]
+        h, v = _c128(h_rkb, n_rkb * n_rkb, "h_rkb"), _c128(v_total, nu * n_rkb, "v_total")
+        s, x = _c128(s_full, n_rkb * n_rkb, "s_full"), _c128(x_full, n_rkb * nx, "x_full")
+        d = _c128(density, n_rkb * n_rkb, "density")
         f, fp = _out(out, (n_rkb, n_rkb), np.complex128)
         self._call("rr_rkb_fock_ri", ctypes.c_int(slot_ll), ctypes.c_int(slot_ss), ctypes.c_int(precision), _sz(nl),
                    _sz(ns), h.ptr, v.ptr, _sz(n_rkb), s.ptr, x.ptr, _sz(x_full.shape[1]), d.ptr, fp)
@@ -293,9 +316,10 @@ class Context:
         """buildRiMoEri -- src/UKB/RiMoEri.cpp:64-86. Returns (n_aux, n_total, n_total) complex."""
         naux = self.tensor_info(slot_ll)[0]
         nt = c_ukb.shape[1]
-        c = _c128(c_ukb)
+        c = _c128(c_ukb, (2 * nl + 2 * ns) * nt, "c_ukb")
         dt = np.complex128 if out_precision == FP64 else np.complex64
         if out is not None:
+            _need(out, naux * nt * nt, "out")
             o, op = out, ctypes.c_void_p(out.data_ptr())
         else:
             o = np.empty((naux, nt, nt), dtype=dt)
@@ -307,7 +331,7 @@ class Context:
     def ri_mo_eri_rotated(self, b, u, precision=FP64, out=None):
         """RiMoEri::rotated -- src/UKB/RiMoEri.cpp:88-110."""
         naux, nt = b.shape[0], u.shape[0]
-        bb, uu = _c128(b), _c128(u)
+        bb, uu = _c128(b, naux * nt * nt, "b"), _c128(u, nt * nt, "u")
         o, op = _out(out, (naux, nt, nt), np.complex128)
         self._call("rr_ri_mo_eri_rotated", ctypes.c_int(precision), _sz(nt), _sz(naux), bb.ptr, uu.ptr, op)
         return o
@@ -325,7 +349,7 @@ class Context:
         """buildRiNonRelSpinMoEri -- src/NON_REL/NonRelSpinRiMoEri.cpp:13-43."""
         naux, n, _ = self.tensor_info(slot)
         nmo = c_spatial.shape[1]
-        c = _f64(c_spatial)
+        c = _f64(c_spatial, n * nmo, "c_spatial")
         o, op = _out(out, (naux, nmo, nmo), np.float64)
         self._call("rr_build_ri_nonrel_spin_mo_eri", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), c.ptr,
                    _sz(nmo), op)
@@ -334,7 +358,7 @@ class Context:
     def ri_nonrel_spin_mo_eri_rotated(self, b, u, precision=FP64, out=None):
         """RiNonRelSpinMoEri::rotated -- src/NON_REL/NonRelSpinRiMoEri.cpp:45-95."""
         naux, ns = b.shape[0], u.shape[0] // 2
-        bb, uu = _f64(b), _f64(u)
+        bb, uu = _f64(b, naux * ns * ns, "b"), _f64(u, 4 * ns * ns, "u")
         o, op = _out(out, (naux, ns, ns), np.float64)
         self._call("rr_ri_nonrel_spin_mo_eri_rotated", ctypes.c_int(precision), _sz(ns), _sz(naux), bb.ptr, uu.ptr, op)
         return o
@@ -351,7 +375,7 @@ class Context:
     def nonrel_fock_cholesky(self, slot, h_core, p, precision=FP64, out=None):
         """nonRelFockMatrix(h, AoCholesky, P) -- src/Utils/AoCholesky.cpp:108-149."""
         n = h_core.shape[0]
-        h, pp = _f64(h_core), _f64(p)
+        h, pp = _f64(h_core, n * n, "h_core"), _f64(p, n * n, "p")
         f, fp = _out(out, (n, n), np.float64)
         self._call("rr_nonrel_fock_cholesky", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), h.ptr, pp.ptr, fp)
         return f
@@ -359,7 +383,7 @@ class Context:
     def x2c_fock_cholesky(self, slot, h_x2c, p, precision=FP64, out=None):
         """x2cFockMatrix(h, AoCholesky, P) -- src/Utils/AoCholesky.cpp:151-217."""
         n2 = h_x2c.shape[0]
-        h, pp = _c128(h_x2c), _c128(p)
+        h, pp = _c128(h_x2c, n2 * n2, "h_x2c"), _c128(p, n2 * n2, "p")
         f, fp = _out(out, (n2, n2), np.complex128)
         self._call("rr_x2c_fock_cholesky", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n2 // 2), h.ptr, pp.ptr,
                    fp)
@@ -369,7 +393,7 @@ class Context:
         """Spatial block of aoCholeskyToMoSpinOrbital -- src/Utils/AoCholesky.cpp:219-243."""
         nchol, n, _ = self.tensor_info(slot)
         nmo = c.shape[1]
-        cc = _f64(c)
+        cc = _f64(c, n * nmo, "c")
         o, op = _out(out, (nchol, nmo, nmo), np.float64)
         self._call("rr_cholesky_to_mo_spatial", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), cc.ptr, _sz(nmo),
                    op)
@@ -379,7 +403,7 @@ class Context:
         """aoCholeskyToMoSpinor -- src/Utils/AoCholesky.cpp:270-295."""
         nchol, n, _ = self.tensor_info(slot)
         nmo = c.shape[1]
-        cc = _c128(c)
+        cc = _c128(c, 2 * n * nmo, "c")
         o, op = _out(out, (nchol, nmo, nmo), np.complex128)
         self._call("rr_cholesky_to_mo_spinor", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), cc.ptr, _sz(nmo),
                    op)
@@ -398,10 +422,11 @@ class Context:
         dt = np.complex128 if cplx else np.float64
         n = h_mo.shape[0]
         naux = b.shape[0]
-        bb, hh = _Arr(b, dt), _Arr(h_mo, dt)
-        oc, h2, x2 = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        bb, hh = _Arr(b, dt, naux * n * n, "b"), _Arr(h_mo, dt, n * n, "h_mo")
+        oc, h2, x2 = _f64(occupations, n, "occupations"), _f64(two_rdm_h, n * n, "two_rdm_h"), _f64(two_rdm_x, n * n, "two_rdm_x")
         pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
-        l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+        _need(pa, n, "pair_of")
+        l1, l2 = _f64(two_rdm_l1, n * n, "two_rdm_l1"), _f64(two_rdm_l2, n * n, "two_rdm_l2")
         o, op = _out(out, (n, n), dt)
         self._call("rr_ri_hx_fock_gemm", ctypes.c_int(int(cplx)), _sz(n), _sz(naux), bb.ptr, hh.ptr, oc.ptr, h2.ptr,
                    x2.ptr, ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr, op)
@@ -475,15 +500,16 @@ class Context:
         n, naux = h_mo.shape[0], b.shape[0]
         pp = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint64).reshape(-1, 2))
         npairs = pp.shape[0]
-        vv = _f64(v)
+        vv = _f64(v, npairs * (2 if cplx else 1), "v")
         w = np.zeros(npairs * (2 if cplx else 1))
-        hh, bb = _Arr(h_mo, dt), _Arr(b, dt)
-        oc, hc, xc = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        hh, bb = _Arr(h_mo, dt, n * n, "h_mo"), _Arr(b, dt, naux * n * n, "b")
+        oc, hc, xc = _f64(occupations, n, "occupations"), _f64(two_rdm_h, n * n, "two_rdm_h"), _f64(two_rdm_x, n * n, "two_rdm_x")
         args = [_sz(n), hh.ptr, bb.ptr, _sz(naux), oc.ptr, hc.ptr, xc.ptr]
         if hx:
-            ff = _Arr(fock, dt)
+            ff = _Arr(fock, dt, n * n, "fock")
             pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
-            l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+            _need(pa, n, "pair_of")
+            l1, l2 = _f64(two_rdm_l1, n * n, "two_rdm_l1"), _f64(two_rdm_l2, n * n, "two_rdm_l2")
             args += [ff.ptr, ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr]
         args += [_sz(npairs), ctypes.c_void_p(pp.ctypes.data), vv.ptr, ctypes.c_void_p(w.ctypes.data)]
         self._call(name, *args)
@@ -519,7 +545,7 @@ class Context:
         n2 = int(np.prod(raw.shape[1:]))
         n = int(round(np.sqrt(n2)))
         kept = metric_x.shape[1]
-        r, x = _f64(raw), _f64(metric_x)
+        r, x = _f64(raw), _f64(metric_x, naux_raw * kept, "metric_x")
         o, op = _out(out, (kept, n2), np.float64)
         self._call("rr_ri_three_center_tensor", _sz(n), _sz(naux_raw), r.ptr, x.ptr, _sz(kept), op)
         return o
@@ -528,7 +554,7 @@ class Context:
         """transformRiThreeCenterAoLegs -- src/AO_ints/ThreeCenterIntegrals.cpp:191-217 (t^T B_P t)."""
         naux = eri3.shape[0]
         n_out = t.shape[1]
-        e, tt = _f64(eri3), _f64(t)
+        e, tt = _f64(eri3, naux * n_ao * n_ao, "eri3"), _f64(t, n_ao * n_out, "t")
         o, op = _out(out, (naux, n_out * n_out), np.float64)
         self._call("rr_transform_ri_three_center_ao_legs", _sz(naux), _sz(n_ao), e.ptr, tt.ptr, _sz(n_out), op)
         return o
@@ -536,7 +562,7 @@ class Context:
     def canonical_orthogonalize(self, s, threshold=1e-10):
         """canonicalOrthogonalize -- src/Linear_Algebra/LinearAlgebra.cpp:267-310. Returns X (n x n_kept)."""
         n = s.shape[0]
-        ss = _f64(s)
+        ss = _f64(s, n * n, "s")
         x = np.zeros(n * n)
         kept = _sz()
         self._call("rr_canonical_orthogonalize", _sz(n), ss.ptr, ctypes.c_double(threshold),
@@ -548,10 +574,13 @@ class Context:
                        pair_of=None, two_rdm_l1=None, two_rdm_l2=None, out=None):
         """riJkOnlyFockMatrix / riPnofFockMatrix body -- src/Hessian_opt/RiJkOnlyFock.cpp:14-33."""
         n_rkb, n_mo = c_rkb.shape
-        hh, vv, cc = _c128(h_rkb), _c128(v_total), _c128(c_rkb)
-        oc, h2, x2 = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        hh, vv = _c128(h_rkb, n_rkb * n_rkb, "h_rkb"), _c128(v_total, (2 * nl + 2 * ns) * n_rkb, "v_total")
+        cc = _c128(c_rkb)
+        oc, h2 = _f64(occupations, n_mo, "occupations"), _f64(two_rdm_h, n_mo * n_mo, "two_rdm_h")
+        x2 = _f64(two_rdm_x, n_mo * n_mo, "two_rdm_x")
         pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
-        l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+        _need(pa, n_mo, "pair_of")
+        l1, l2 = _f64(two_rdm_l1, n_mo * n_mo, "two_rdm_l1"), _f64(two_rdm_l2, n_mo * n_mo, "two_rdm_l2")
         o, op = _out(out, (n_mo, n_mo), np.complex128)
         self._call("rr_ri_mo_fock_rkb", ctypes.c_int(slot_ll), ctypes.c_int(slot_ss), _sz(nl), _sz(ns), hh.ptr, vv.ptr,
                    _sz(n_rkb), cc.ptr, _sz(n_mo), oc.ptr, h2.ptr, x2.ptr,
@@ -562,10 +591,13 @@ class Context:
                           two_rdm_l1=None, two_rdm_l2=None, out=None):
         """riJkOnlyFockMatrixNonRel / riPnofFockMatrixNonRel body -- RiNonRelFullOptFock.cpp:53-98."""
         ns = h_core_ao.shape[0]
-        hh, cc = _f64(h_core_ao), _f64(c_current)
-        oc, h2, x2 = _f64(occupations), _f64(two_rdm_h), _f64(two_rdm_x)
+        nt = 2 * ns  # spin orbitals: c_current is n x 2n (RiNonRelFullOptFock.cpp:53-98)
+        hh, cc = _f64(h_core_ao, ns * ns, "h_core_ao"), _f64(c_current, ns * nt, "c_current")
+        oc, h2 = _f64(occupations, nt, "occupations"), _f64(two_rdm_h, nt * nt, "two_rdm_h")
+        x2 = _f64(two_rdm_x, nt * nt, "two_rdm_x")
         pa = None if pair_of is None else np.ascontiguousarray(np.asarray(pair_of, dtype=np.uint64))
-        l1, l2 = _f64(two_rdm_l1), _f64(two_rdm_l2)
+        _need(pa, nt, "pair_of")
+        l1, l2 = _f64(two_rdm_l1, nt * nt, "two_rdm_l1"), _f64(two_rdm_l2, nt * nt, "two_rdm_l2")
         o, op = _out(out, (2 * ns, 2 * ns), np.float64)
         self._call("rr_ri_mo_fock_nonrel", ctypes.c_int(slot), _sz(ns), hh.ptr, cc.ptr, oc.ptr, h2.ptr, x2.ptr,
                    ctypes.c_void_p(0 if pa is None else pa.ctypes.data), l1.ptr, l2.ptr, op)
@@ -622,7 +654,9 @@ class Context:
         """J, K of D_m = L_m L_m^T for every molecule in one graph launch. Returns ([J_m], [K_m], replayed)."""
         handle, ns = batch
         nm = len(ns)
-        ls = [_f64(l) for l in lfacs]
+        if len(lfacs) != nm:
+            raise ValueError(f"batch_jk_factor: {nm} molecules, {len(lfacs)} factors")
+        ls = [_f64(l, n * l.shape[1], f"lfacs[{m}]") for m, (l, n) in enumerate(zip(lfacs, ns))]
         r = (_sz * nm)(*[l.shape[1] for l in lfacs])
         lp = (ctypes.c_void_p * nm)(*[a.ptr.value for a in ls])
         if outs is None:
@@ -630,8 +664,8 @@ class Context:
             ks = [np.empty((n, n)) for n in ns]
         else:
             js, ks = outs
-        jp = (ctypes.c_void_p * nm)(*[_Arr(x, np.float64).ptr.value for x in js])
-        kp = (ctypes.c_void_p * nm)(*[_Arr(x, np.float64).ptr.value for x in ks])
+        jp = (ctypes.c_void_p * nm)(*[_Arr(x, np.float64, n * n, "J out").ptr.value for x, n in zip(js, ns)])
+        kp = (ctypes.c_void_p * nm)(*[_Arr(x, np.float64, n * n, "K out").ptr.value for x, n in zip(ks, ns)])
         rep = ctypes.c_int(0)
         self._call("rr_batch_jk_factor", handle, ctypes.c_int(precision), r, lp, jp, kp, ctypes.byref(rep))
         return js, ks, bool(rep.value)

@@ -72,6 +72,44 @@ def build_adapter_test(verbose=False):
     return exe
 
 
+VALIDATE_DIR = os.path.join(HERE, "_build", "validate")
+
+
+def build_validation(verbose=False):
+    """Validation build of the library (-DRR_VALIDATE: launch sites check pointer liveness and GEMM
+    extents instead of launching) linked against the host-memory HIP stand-in of tests/validate, and
+    the driver that replays the benchmark's call sequence through it. Test infrastructure: nothing
+    here runs on or needs a GPU. Returns the path of the replay executable."""
+    os.makedirs(VALIDATE_DIR, exist_ok=True)
+    vdir = os.path.join(ROOT, "tests", "validate")
+    mock = os.path.join(VALIDATE_DIR, "libhipmock.so")
+    cmds = [["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+             os.path.join(vdir, "hip_mock.cpp"), "-o", mock]]
+    flags = ["-O1", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-DRR_VALIDATE", "-Wno-unused-function",
+             "-Wno-unused-variable", "-Wno-unused-parameter"]
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(VALIDATE_DIR, os.path.splitext(os.path.basename(src))[0] + ".o")
+        cmd = [HIPCC] + flags + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd, stderr=subprocess.DEVNULL if not verbose else None), cmd))
+        objs.append(obj)
+    for cmd in cmds:
+        subprocess.run(cmd, check=True)
+    failed = [cmd for p, cmd in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+    vlib = os.path.join(VALIDATE_DIR, "librerdmft_validate.so")
+    subprocess.run(["g++", "-shared", "-fPIC"] + objs + ["-o", vlib, "-L" + VALIDATE_DIR, "-lhipmock",
+                    "-Wl,-rpath," + VALIDATE_DIR], check=True)
+    exe = os.path.join(VALIDATE_DIR, "replay")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    os.path.join(vdir, "replay.cpp"), "-o", exe, "-L" + VALIDATE_DIR, "-lrerdmft_validate",
+                    "-lhipmock", "-Wl,-rpath," + VALIDATE_DIR], check=True)
+    return exe
+
+
 def build_oracle():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 

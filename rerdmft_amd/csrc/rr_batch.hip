@@ -188,6 +188,7 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
       const size_t cnt = b->t[m].n * r[m];
       if (!cnt) continue;
       if (rr::is_device_ptr(lfac[m])) {
+        RR_VALIDATE_EXTENT(lfac[m], cnt * sizeof(double), "batch factor");
         dcopy.push_back({lfac[m], b->lin + loff[m], cnt, 0});
       } else {
         if (host.empty()) host.assign(ltot, 0.0);
@@ -232,9 +233,10 @@ int rr_batch_jk_factor(rr_handle h, rr_batch_t b, int precision, const size_t* r
       double* dst[2] = {j_out[m], k_out[m]};
       const double* src[2] = {b->jout + b->o_off[m], b->kout + b->o_off[m]};
       for (int q = 0; q < 2; ++q) {
-        if (rr::is_device_ptr(dst[q]))
+        if (rr::is_device_ptr(dst[q])) {
+          RR_VALIDATE_EXTENT(dst[q], cnt * sizeof(double), "batch output");
           outs.push_back({src[q], dst[q], cnt, 0});
-        else
+        } else
           RR_HIP(hipMemcpyAsync(dst[q], src[q], cnt * sizeof(double), hipMemcpyDeviceToHost, c.stream));
       }
     }

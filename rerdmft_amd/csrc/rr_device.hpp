@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "rr_validate.hpp"
+
 #include <cstdint>
 
 namespace rr {

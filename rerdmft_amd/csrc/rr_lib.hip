@@ -651,9 +651,87 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
   }
 }
 
+#ifdef RR_VALIDATE
+// Extent check of one GEMM launch (validation build): the highest scalar element each operand and
+// output can touch, from the same index arithmetic as the kernels' loads and epilogues, against the
+// allocation its pointer lies in. k is read in chunks of `kgran` scalars (a fragment of the register
+// kernel, a 16-byte vector of the LDS-staged ones), so a row is read up to K rounded up to a chunk.
+void validate_gemm(const GemmArgs& g, size_t si, size_t so, bool cplx, bool lds, const char* where) {
+  if (g.M <= 0 || g.N <= 0) return;
+  const long long cw = cplx ? 2 : 1;
+  const long long kgran = lds ? (long long)(16 / si) : (si == 8 ? 4 : 8);
+  const long long kblk = (long long)g.K * cw, kread = (kblk + kgran - 1) / kgran * kgran;
+  const long long kb = std::max(1, g.kbatch);
+  if (g.K > 0) {
+    const long long a_end = ((long long)(g.nbatch - 1) * g.sAb + (kb - 1) * g.sAk + (long long)(g.M - 1) * g.lda) * cw + kread;
+    const long long b_end = ((long long)(g.nbatch - 1) * g.sBb + (kb - 1) * g.sBk + (long long)(g.N - 1) * g.ldb) * cw + kread;
+    rrv_check_range(g.A, (unsigned long long)a_end * si, "GEMM operand A", where);
+    rrv_check_range(g.B, (unsigned long long)b_end * si, "GEMM operand B", where);
+  }
+  // Output: the offset of (m, n) is monotone inside each piece cut by mcut / ncut and by the row
+  // and column blocks, so its maximum over a piece sits at the piece's last row and column.
+  auto ends = [](long long lo, long long hi, long long blk, std::vector<long long>& out) {  // [lo, hi)
+    if (hi <= lo) return;
+    out.push_back(hi - 1);
+    if (blk > 0)
+      for (long long e = (lo / blk + 1) * blk - 1; e < hi - 1; e += blk) out.push_back(e);
+  };
+  std::vector<long long> ms, ns;
+  if (g.mcut > 0 && g.mcut < g.M) {
+    ends(0, g.mcut, g.mrow, ms);
+    std::vector<long long> m2;
+    ends(0, g.M - g.mcut, g.mrow, m2);
+    for (long long v : m2) ms.push_back(v + g.mcut);
+  } else {
+    ends(0, g.M, g.mrow, ms);
+  }
+  if (g.ncut > 0 && g.ncut < g.N) {
+    ends(0, g.ncut, g.nblk, ns);
+    std::vector<long long> n2;
+    ends(0, g.N - g.ncut, g.nblk, n2);
+    for (long long v : n2) ns.push_back(v + g.ncut);
+  } else {
+    ends(0, g.N, g.nblk, ns);
+  }
+  const long long zoff = ((long long)(g.nbatch - 1) * g.sCb + (long long)(g.nsplit - 1) * g.sCz) * cw;
+  long long c_end = -1, c2_end = -1;
+  for (long long m : ms)
+    for (long long n : ns) {
+      long long mm = m, nn = n, off = 0;
+      bool second = false;
+      if (g.mcut > 0 && mm >= g.mcut) mm -= g.mcut, second = true;
+      if (g.ncut > 0 && nn >= g.ncut) nn -= g.ncut, second = true;
+      if (g.nblk > 0) {
+        const long long b = nn / g.nblk;
+        nn -= b * g.nblk;
+        off += b * g.sCnb * cw;
+      }
+      if (g.mrow > 0) {
+        const long long b = mm / g.mrow;
+        mm -= b * g.mrow;
+        off += b * g.sCmb;
+      }
+      off += (mm * g.ldcm + nn * g.ldcn) * cw + cw;  // one past the last scalar written
+      long long& e = second ? c2_end : c_end;
+      e = std::max(e, zoff + off);
+    }
+  if (c_end > 0) rrv_check_range(g.C, (unsigned long long)c_end * so, "GEMM output C", where);
+  if (c2_end > 0) rrv_check_range(g.C2, (unsigned long long)c2_end * so, "GEMM second output C2", where);
+  if (g.mrow > 0 && !lds) rrv_check_range(nullptr, 1, "GEMM row blocks on a kernel without them", where);
+}
+#endif
+
 void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
   RR_REQUIRE(sp.cfg >= 0 && sp.cfg < kNumTiles, RR_ERR_ARG, "bad tile configuration");
   const bool use_lds = sp.lds >= 0 && !sp.cplx && g.kbatch == 0;
+#ifdef RR_VALIDATE
+  {
+    char where[160];
+    snprintf(where, sizeof(where), "run_gemm M=%d N=%d K=%d nbatch=%d nsplit=%d kbatch=%d prec=%d out=%d cplx=%d lds=%d",
+             g.M, g.N, g.K, g.nbatch, g.nsplit, g.kbatch, sp.prec, sp.out, (int)sp.cplx, use_lds ? sp.lds : -1);
+    validate_gemm(g, prec_size(sp.prec), prec_size(sp.out), sp.cplx, use_lds, where);
+  }
+#endif
   RR_REQUIRE(!use_lds || sp.lds < kNumLds, RR_ERR_ARG, "bad LDS tile configuration");
   const TileCfg t = kTiles[sp.cfg];
   const int BM = use_lds ? kLds[sp.lds].bm : 32 * t.wm, BN = use_lds ? kLds[sp.lds].bn : 32 * t.wn;
@@ -1426,6 +1504,9 @@ OutBuf out_buf(Context& c, int id, double* user, size_t count) {
   o.user = user, o.count = count;
   if (!user) return o;
   if (is_device_ptr(user)) {
+#ifdef RR_VALIDATE
+    rrv_check_range(user, count * sizeof(double), "caller output array (declared extent)", "out_buf");
+#endif
     o.dev = user;
   } else {
     o.dev = static_cast<double*>(wsbuf(c, id, std::max<size_t>(1, count) * sizeof(double)));
@@ -1438,7 +1519,12 @@ void finish_out(Context& c, const OutBuf& o) {
 }
 const double* in_buf(Context& c, int id, const double* user, size_t count) {
   if (!user) return nullptr;
-  if (is_device_ptr(user)) return user;
+  if (is_device_ptr(user)) {
+#ifdef RR_VALIDATE
+    rrv_check_range(user, count * sizeof(double), "caller input array (declared extent)", "in_buf");
+#endif
+    return user;
+  }
   return to_device<double>(c, id, user, count);
 }
 
@@ -1534,7 +1620,7 @@ __global__ void k_zero_strict_lower(double* a, size_t n, int cw) {
 namespace {
 constexpr int kFroBlocks = 256;
 
-double fro2(Context& c, const double* t, size_t n, int cw, bool sub_identity) {
+double fro2(Context& c, const double* t, size_t n, int cw, bool sub_identity, const char* tag) {
   double* part = static_cast<double*>(wsbuf(c, WS_T4 + 800, kFroBlocks * sizeof(double)));
   hipLaunchKernelGGL(k_fro_partial, dim3(kFroBlocks), dim3(256), 0, c.stream, t, n, cw, (int)sub_identity, part);
   RR_HIP(hipGetLastError());
@@ -1543,6 +1629,8 @@ double fro2(Context& c, const double* t, size_t n, int cw, bool sub_identity) {
   RR_HIP(hipStreamSynchronize(c.stream));
   double s = 0.0;
   for (double v : h) s += v;
+  RR_CONTROL(tag, s);
+  (void)tag;
   return s;
 }
 
@@ -1600,9 +1688,9 @@ long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool we
   RR_HIP(hipGetLastError());
   dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // A^2
   dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, T, nx, T, nx, X, nx);  // A^4
-  const double a4 = std::sqrt(fro2(c, X, nx, cw, false));
+  const double a4 = std::sqrt(fro2(c, X, nx, cw, false, "projector_scale"));
   dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // A^8
-  const double a8 = std::sqrt(fro2(c, T, nx, cw, false));
+  const double a8 = std::sqrt(fro2(c, T, nx, cw, false, "projector_scale"));
   if (!(a8 > 0.0) || !std::isfinite(a8) || !(a4 > 0.0) || !std::isfinite(a4)) return -1;
   const double hi = std::pow(a8, 0.125), lo = std::pow(a8 / a4, 0.25);
   const double s = std::max(lo, 1.01 * hi / std::sqrt(3.0));
@@ -1612,7 +1700,7 @@ long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool we
   bool done = false;
   for (int it = 0; it < kMaxSteps && !done; ++it) {
     dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, X, nx, T, nx);  // X^2
-    const double e = std::sqrt(fro2(c, T, nx, cw, true));           // ||X^2 - I||_F
+    const double e = std::sqrt(fro2(c, T, nx, cw, true, "projector_residual"));  // ||X^2 - I||_F
     if (!std::isfinite(e) || e > 4.0 * nx) return -1;
     done = e < 1e-9;  // one more step leaves an error at rounding level
     dgemm_small(c, cplx, 'N', 'N', nx, nx, nx, X, nx, T, nx, X, nx, -0.5, 1.5);  // X (3 - X^2) / 2
@@ -1625,6 +1713,7 @@ long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool we
   double trace = 0.0;
   RR_HIP(hipMemcpyAsync(&trace, tr, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));
+  RR_CONTROL("projector_trace", trace);
   const double rr_ = std::nearbyint(trace);
   if (!(std::fabs(trace - rr_) < 1e-6) || rr_ < 0.0 || rr_ > (double)nx) return -1;
   const size_t r = (size_t)rr_;
@@ -1648,7 +1737,7 @@ long projector_factor(Context& c, bool cplx, size_t nx, const double* M, bool we
   // a product, a factorization and a triangular solve).
   for (int it = 0;; ++it) {
     dgemm_small(c, cplx, ct, 'N', r, r, nx, Y, r, Y, r, G, r);  // S = Q^H Q
-    const double e = std::sqrt(fro2(c, G, r, cw, true));          // ||S - I||_F
+    const double e = std::sqrt(fro2(c, G, r, cw, true, "projector_residual"));  // ||S - I||_F
     if (!(e < 0.1) || it == 4) return -1;  // also a failed factorization (NaN)
     dgemm_small(c, cplx, 'N', 'N', nx, r, r, Y, r, G, r, om, r);  // Q S
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(nx * r * cw)), dim3(256), 0, c.stream, nx * r * cw, 1.5, Y, -0.5, om, Y);
@@ -2550,6 +2639,23 @@ void cholesky_exchange(Context& c, const Tensor& t, int prec, const double* dPop
   g.M = (int)n, g.N = (int)(cplx_rows ? 2 * n : n), g.K = (int)ld, g.kbatch = (int)nchol;
   GemmSpec sp;
   sp.prec = op, sp.out = P64, sp.cfg = 2;
+  // The output alone gives few 128 x 128 tiles (16 at n = 500) for a contraction over n_chol * ld:
+  // split the flat k into up to 2048 workgroups' worth of partial sums (at most 512 MB of them),
+  // summed in split order afterwards. Each split covers whole k-slabs (kper a multiple of the slab).
+  const size_t tiles = ((n + 127) / 128) * (((size_t)g.N + 127) / 128);
+  const size_t ktot = nchol * ld, slab = op == P64 ? 16 : 32, outsz = n * (size_t)g.N;
+  const size_t ns = std::min<size_t>({256, std::max<size_t>(1, 2048 / tiles), std::max<size_t>(1, ktot / 4096),
+                                      std::max<size_t>(1, (size_t(512) << 20) / (outsz * sizeof(double)))});
+  if (ns > 1 && ldk == (size_t)g.N) {
+    const size_t kper = roundup((ktot + ns - 1) / ns, slab);
+    const int nsplit = (int)((ktot + kper - 1) / kper);
+    double* part = static_cast<double*>(wsbuf(c, WS_KPART, (size_t)nsplit * outsz * sizeof(double)));
+    g.C = part, g.sCz = (long long)outsz, g.kper = (int)kper, g.nsplit = nsplit;
+    gemm_auto(c, sp, g);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(grid_for(outsz)), dim3(256), 0, c.stream, part, nsplit, outsz, outsz, kout);
+    RR_HIP(hipGetLastError());
+    return;
+  }
   gemm_auto(c, sp, g);
 }
 
@@ -2671,6 +2777,7 @@ void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t
   if (naux == 0 || n == 0) return;
   const bool dev = is_device_ptr(eri3);
   const size_t rowbytes = n * n * sizeof(double);
+  if (dev) RR_VALIDATE_EXTENT(eri3, naux * rowbytes, "3-index tensor source");
   size_t chunk = std::max<size_t>(1, std::min<size_t>(naux, (size_t(256) << 20) / rowbytes));
   chunk = std::min<size_t>(chunk, 65535);
   double* stage = dev ? nullptr : static_cast<double*>(wsbuf(c, WS_STAGE, chunk * rowbytes));
@@ -2695,6 +2802,11 @@ void tensor_store(Context& c, Tensor& t, const double* eri3, size_t naux, size_t
   int h_asym[2] = {0, 0};
   RR_HIP(hipMemcpyAsync(h_asym, asym, sizeof(h_asym), hipMemcpyDeviceToHost, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));
+#ifdef RR_VALIDATE
+  h_asym[1] = (int)naux;  // no kernel ran: every row is taken as non-zero unless the replay says otherwise
+#endif
+  RR_CONTROL("upload_asymmetric", h_asym[0]);
+  RR_CONTROL("upload_nonzero_rows", h_asym[1]);
   t.symmetric = h_asym[0] == 0;
   t.naux_nz = (size_t)h_asym[1];
 }
@@ -3598,6 +3710,7 @@ int rr_build_ri_mo_eri(rr_handle h, int slot_ll, int slot_ss, int precision, int
     const size_t count = 2 * tl.naux * n_total * n_total;
     const size_t esz = out_precision == RR_FP64 ? 8 : 4;
     const bool dev = rr::is_device_ptr(out);
+    if (dev) RR_VALIDATE_EXTENT(out, count * esz, "buildRiMoEri output");
     void* dout = dev ? out : rr::wsbuf(c, rr::WS_MO_OUT, std::max<size_t>(1, count) * esz);
     if (precision == RR_FP64) rr::mo_cplx_slot<double, double>(c, fl, precision, dC, n_total, nu, (double*)dout);
     else if (precision == RR_FP32) rr::mo_cplx_slot<float, float>(c, fl, precision, dC, n_total, nu, (float*)dout);
