@@ -33,6 +33,27 @@ SEED = 20261203 + 1
 PEAK = {"hbm": 8000.0, "f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # GB/s, TFLOP/s (dense)
 
 
+def progress(msg):
+    """One line on stderr per finished step of the run (the JSON result goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def legs(a, torch, rr, ctx):
+    out = {}
+    for key, on, fn in (("batch_uneven_molecules", a.batch_molecules > 0, lambda: batch_leg(torch, rr, ctx, a.batch_molecules)),
+                        ("sigma_trial_densities", a.sigma_densities > 0, lambda: sigma_leg(torch, rr, ctx, a.sigma_densities)),
+                        ("relativistic", bool(a.relativistic), lambda: relativistic_leg(torch, rr, ctx, cpu=a.cpu_sample > 0)),
+                        ("other_rows", bool(a.rows), lambda: rows_leg(torch, rr, ctx))):
+        if not on:
+            out[key] = None
+            continue
+        t0 = time.time()
+        progress(f"{key}: start")
+        out[key] = fn()
+        progress(f"{key}: done in {time.time() - t0:.1f} s")
+    return out
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -256,9 +277,11 @@ def main():
                 fp64_kernels[k2[1]] = {"avg_ms": avg, "GB/s": w2 / (avg * 1e-3) / 1e9,
                                        "frac": w2 / (avg * 1e-3) / 1e9 / PEAK["hbm"]}
 
+    progress(f"headline sequence timed ({rate:.1f} builds/s)")
     cpu = None
     if a.cpu_sample > 0:
         cpu = cpu_baseline(n, nocc, min(a.cpu_sample, naux), naux, cs[0])
+        progress("CPU baseline timed")
 
     out = {
         "metric": "RI J+K Fock builds/sec (mixed vs FP64) at |dE|<=1e-8 Ha; % HBM/MFMA peak",
@@ -289,11 +312,8 @@ def main():
         "phase_ms_mixed": {f"{k[1]}[{k[0]}]": round(v[0], 3) for k, v in phases_p.items() if v[1]},
         "cpu_baseline": cpu,
         "setup_s": setup_s,
-        "batch_uneven_molecules": batch_leg(torch, rr, ctx, a.batch_molecules) if a.batch_molecules > 0 else None,
-        "sigma_trial_densities": sigma_leg(torch, rr, ctx, a.sigma_densities) if a.sigma_densities > 0 else None,
-        "relativistic": relativistic_leg(torch, rr, ctx, cpu=a.cpu_sample > 0) if a.relativistic else None,
-        "other_rows": rows_leg(torch, rr, ctx) if a.rows else None,
     }
+    out.update(legs(a, torch, rr, ctx))
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
@@ -450,6 +470,7 @@ def rows_leg(torch, rr, ctx, reps=3):
     out = {}
 
     def row(name, fn, flop=None, **info):
+        progress(f"row {name}")
         try:
             ms = 1e3 * _timed(torch, fn, reps)
             r = {"ms_per_call": round(ms, 3), **info}
