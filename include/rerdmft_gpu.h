@@ -90,6 +90,17 @@ int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const dou
 int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double* density, const double* lfac,
                     const double* rfac, size_t r, double* j_out, double* k_out);
 
+/* Allocation-free variant of rr_ri_jk_factor for D = L L^T (density NULL, rfac NULL; J from the
+ * half-transformed factor). No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid it). It
+ * calls no hipMalloc / hipFree and no allocating library routine: every workspace lives in one
+ * static 160 MiB scratch region per device (held until the process ends). lfac, j_out and k_out
+ * must be device memory; a shape whose workspace exceeds the region is rejected with
+ * RR_ERR_ARG before any launch. Results equal rr_ri_jk_factor's. Every call on a device shares that
+ * device's region, so calls that overlap on one GPU (other threads, other handles) race. */
+int rr_ri_jk_factor_noalloc(rr_handle h, int slot, int precision, size_t n, const double* lfac, size_t r,
+                            double* j_out, double* k_out);
+
 /* Many densities sharing the tensor and the left factor (no reference counterpart; the trial
  * densities of the orbital-Hessian sigma products, D_k = L R_k^T with L the occupied orbitals):
  * for k < n_dens, J_k = sum_P tr(B_P D_k) B_P and K_k = sum_P B_P^T (L R_k^T) B_P, as n_dens calls of

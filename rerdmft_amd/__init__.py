@@ -25,7 +25,7 @@ PRECISION_NAMES = {FP64: "f64", FP32: "f32", BF16: "bf16"}
 EXPORTED = [
     "rr_create", "rr_destroy", "rr_last_error", "rr_set_tile_config", "rr_set_recovery_method", "rr_timing",
     "rr_tensor_upload", "rr_tensor_generate", "rr_tensor_info", "rr_tensor_release",
-    "rr_nonrel_fock_ri", "rr_ri_jk_factor", "rr_ri_jk_multi", "rr_ri_jk_delta", "rr_x2c_fock_ri",
+    "rr_nonrel_fock_ri", "rr_ri_jk_factor", "rr_ri_jk_factor_noalloc", "rr_ri_jk_multi", "rr_ri_jk_delta", "rr_x2c_fock_ri",
     "rr_ri_fock_two_electron_direct", "rr_rkb_fock_ri", "rr_build_ri_mo_eri", "rr_ri_mo_eri_rotated",
     "rr_ri_mo_eri_positive_block", "rr_build_ri_nonrel_spin_mo_eri", "rr_ri_nonrel_spin_mo_eri_rotated",
     "rr_ri_nonrel_spin_embedded_b", "rr_nonrel_fock_cholesky", "rr_x2c_fock_cholesky",
@@ -248,6 +248,19 @@ class Context:
         k, kp = _out(k_out, (n, n), np.float64) if want_k else (None, ctypes.c_void_p(0))
         self._call("rr_ri_jk_factor", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), d.ptr, l.ptr, rr_.ptr,
                    _sz(r), jp, kp)
+        return j, k
+
+    def ri_jk_factor_noalloc(self, slot, lfac, precision=FP64, j_out=None, k_out=None):
+        """Allocation-free J, K of D = L L^T (rr_ri_jk_factor_noalloc): lfac, j_out and k_out are
+        device tensors; the workspace is the library's static per-device scratch region."""
+        n, r = lfac.shape
+        l = _f64(lfac)
+        if not (l.torch and j_out is not None and k_out is not None):
+            raise ValueError("ri_jk_factor_noalloc: lfac, j_out and k_out must be device tensors")
+        j, jp = _out(j_out, (n, n), np.float64)
+        k, kp = _out(k_out, (n, n), np.float64)
+        self._call("rr_ri_jk_factor_noalloc", ctypes.c_int(slot), ctypes.c_int(precision), _sz(n), l.ptr, _sz(r), jp,
+                   kp)
         return j, k
 
     def ri_jk_multi(self, slot, lfac, rfacs, precision=FP64, want_j=True, want_k=True, j_out=None, k_out=None):

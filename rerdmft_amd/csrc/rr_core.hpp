@@ -84,6 +84,25 @@ struct Context {
   // converge), 1 eigensolver only; counts of the recoveries each method finished.
   int recovery_method = 0;
   long long n_projector = 0, n_eigensolver = 0;
+  // Allocation-free calls: while `arena` is set, workspace requests are carved from it in request
+  // order (256-byte aligned, a region per id and size) instead of allocated; a request past
+  // arena_size is an error. Set only for the duration of such a call (ArenaScope).
+  char* arena = nullptr;
+  size_t arena_size = 0, arena_used = 0;
+  std::map<int, std::pair<void*, size_t>> arena_ws;
+};
+
+// Routes the context's workspace requests to a caller-provided device region for one call.
+struct ArenaScope {
+  Context& c;
+  ArenaScope(Context& cc, void* base, size_t size) : c(cc) {
+    c.arena = static_cast<char*>(base), c.arena_size = size, c.arena_used = 0;
+    c.arena_ws.clear();
+  }
+  ~ArenaScope() {
+    c.arena = nullptr, c.arena_size = 0, c.arena_used = 0;
+    c.arena_ws.clear();
+  }
 };
 
 struct Error {

@@ -26,7 +26,24 @@
 
 namespace rr {
 // Named grow-only device workspace buffers.
+// Scratch region of the allocation-free entry point (defined with it below).
+hipError_t register_scratch(int device);
+
 void* wsbuf(Context& c, int id, size_t bytes) {
+  if (c.arena) {
+    auto at = c.arena_ws.find(id);
+    if (at != c.arena_ws.end() && at->second.second >= bytes) return at->second.first;
+    const size_t off = roundup(c.arena_used, 256), sz = std::max<size_t>(bytes, 256);
+    if (off + sz > c.arena_size) {
+      char msg[200];
+      snprintf(msg, sizeof(msg), "allocation-free call: the workspace needs more than the %zu MiB scratch region",
+               c.arena_size >> 20);
+      throw Error{RR_ERR_ARG, msg};
+    }
+    c.arena_used = off + sz;
+    c.arena_ws[id] = {c.arena + off, sz};
+    return c.arena + off;
+  }
   auto it = c.ws.find(id);
   if (it != c.ws.end() && it->second.second >= bytes) return it->second.first;
   if (it != c.ws.end()) {
@@ -811,6 +828,10 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
     run_gemm(c, sp, g);
     return;
   }
+  if (c.arena) {  // allocation-free call: no timing events, the call site's default tile
+    run_gemm(c, sp, g);
+    return;
+  }
   std::vector<std::pair<int, int>> cand;
   for (int cfg : {1, 2, 5}) {
     const TileCfg t = kTiles[cfg];
@@ -1004,12 +1025,13 @@ struct PhaseTimer {
   // calling thread, and the totals and the pending event pairs are guarded by `mu`.
   static std::atomic<bool> enabled;
   static thread_local int cur_prec;  // precision of the build in progress (set by the entry points)
+  static thread_local bool suppressed;  // no events in allocation-free calls
   static double ms[24];
   static long long cnt[24];
   static std::vector<Pending> pending;
   static std::mutex mu;
   PhaseTimer(Context& cc, int ph) : c(cc), phase(ph + 8 * cur_prec) {
-    if (!enabled) return;
+    if (!enabled || suppressed) return;
     RR_HIP(hipEventCreate(&a));
     RR_HIP(hipEventCreate(&b));
     RR_HIP(hipEventRecord(a, c.stream));
@@ -1044,6 +1066,7 @@ struct PhaseTimer {
 };
 std::atomic<bool> PhaseTimer::enabled{false};
 thread_local int PhaseTimer::cur_prec = 0;
+thread_local bool PhaseTimer::suppressed = false;
 double PhaseTimer::ms[24] = {0};
 long long PhaseTimer::cnt[24] = {0};
 std::vector<PhaseTimer::Pending> PhaseTimer::pending;
@@ -3254,6 +3277,11 @@ int rr_create(int device, rr_handle* out) {
     delete h;
     return RR_ERR_HIP;
   }
+  if (rr::register_scratch(device) != hipSuccess) {
+    (void)hipStreamDestroy(h->ctx.stream);
+    delete h;
+    return RR_ERR_HIP;
+  }
   if (hipMalloc(&h->ctx.zpage, 256) != hipSuccess || hipMemset(h->ctx.zpage, 0, 256) != hipSuccess) {
     (void)hipStreamDestroy(h->ctx.stream);
     delete h;
@@ -3408,58 +3436,123 @@ int rr_tensor_info(rr_handle h, int slot, size_t* n_aux, size_t* n, unsigned* co
   });
 }
 
+}  // extern "C"
+
+namespace rr {
+// Body of rr_ri_jk_factor (shared with its allocation-free variant, which routes the workspace
+// requests to the static scratch region).
+void ri_jk_factor_body(Context& c, int slot, int precision, size_t n, const double* density, const double* lfac,
+                       const double* rfac, size_t r, double* j_out, double* k_out) {
+  check_prec(precision);
+  PrecScope prec_scope(precision);
+  Tensor& t = get_slot(c, slot);
+  RR_REQUIRE(t.n == n, RR_ERR_DIM, "rr_ri_jk_factor: density dimension does not match the tensor");
+  double* jpart = nullptr;
+  int jsplit = 0;
+  const bool fused_j = j_out && !density;  // J of D = L R^T from the half-transformed factors
+  RR_REQUIRE(!fused_j || (lfac && r > 0 && k_out), RR_ERR_ARG,
+             "rr_ri_jk_factor: J without a density needs the factor and the exchange build");
+  if (j_out && density) {
+    const double* dD = in_buf(c, WS_P, density, n * n);
+    jsplit = coulomb_real_prec(c, t, precision, dD, &jpart, t.symmetric);
+  }
+  ExchangeOut ex;
+  if (k_out && r > 0) {
+    RR_REQUIRE(lfac, RR_ERR_ARG, "rr_ri_jk_factor: K requested without a factor");
+    const double* dL = in_buf(c, WS_CA, lfac, n * r);
+    const double* dR = rfac ? in_buf(c, WS_CB, rfac, n * r) : nullptr;
+    // Lt = L^T (r x n)
+    double* lt = static_cast<double*>(wsbuf(c, WS_T1, 2 * r * n * sizeof(double)));
+    hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(r * n)), dim3(256), 0, c.stream, dL, r, 1, r, r, n, n, 1.0,
+                       lt);
+    if (dR)
+      hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(r * n)), dim3(256), 0, c.stream, dR, r, 1, r, r, n, n,
+                         1.0, lt + r * n);
+    RR_HIP(hipGetLastError());
+    double* dfac = nullptr;
+    double* gamma = nullptr;
+    if (fused_j) {
+      const size_t rp = roundup(r, 8);
+      const int nblk = dR ? 2 : 1;
+      dfac = static_cast<double*>(wsbuf(c, WS_DENS2, n * nblk * rp * sizeof(double)));
+      hipLaunchKernelGGL(k_build_dfac, dim3(grid_for(n * nblk * rp)), dim3(256), 0, c.stream,
+                         dR ? lt + r * n : lt, (const double*)nullptr, r, n, n, rp, nblk, 1.0, 0.0, dfac);
+      RR_HIP(hipGetLastError());
+      gamma = static_cast<double*>(wsbuf(c, WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
+    }
+    if (fused_j)
+      ex = exchange_and_coulomb(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma, true, t.symmetric,
+                              &jpart, &jsplit);
+    else
+      ex = exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma);
+  }
+  OutBuf oj = out_buf(c, WS_OUT_J, j_out, n * n), ok = out_buf(c, WS_OUT_K, k_out, n * n);
+  if (k_out && r == 0) hipLaunchKernelGGL(k_zero, dim3(grid_for(n * n)), dim3(256), 0, c.stream, ok.dev, n * n);
+  real_assemble(c, t, jpart, jsplit, (k_out && r > 0) ? &ex : nullptr, -0.5, nullptr, nullptr, oj.dev,
+                (k_out && r > 0) ? ok.dev : nullptr, t.symmetric);
+  finish_out(c, oj);
+  finish_out(c, ok);
+}
+
+// Scratch region of the allocation-free entry point: one per device (each GPU the library runs on
+// holds it until the process ends). Host code uses only the address obtained per device with
+// hipGetSymbolAddress at context creation, never the array's own name.
+constexpr size_t kNoAllocScratchBytes = size_t(160) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Records the scratch address of the current device (once per device).
+hipError_t register_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[device] = p;
+  return e;
+}
+void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace rr
+
+extern "C" {
+
 int rr_ri_jk_factor(rr_handle h, int slot, int precision, size_t n, const double* density, const double* lfac,
                     const double* rfac, size_t r, double* j_out, double* k_out) {
+  return guarded(h, [&](Context& c) { rr::ri_jk_factor_body(c, slot, precision, n, density, lfac, rfac, r, j_out, k_out); });
+}
+
+int rr_ri_jk_factor_noalloc(rr_handle h, int slot, int precision, size_t n, const double* lfac, size_t r,
+                            double* j_out, double* k_out) {
   return guarded(h, [&](Context& c) {
+    void* base = rr::scratch_ptr(c.device);
+    RR_REQUIRE(base, RR_ERR_HIP, "rr_ri_jk_factor_noalloc: no scratch region on this device");
+    RR_REQUIRE(lfac && j_out && k_out && r > 0, RR_ERR_ARG, "rr_ri_jk_factor_noalloc: needs lfac (r > 0), j_out and k_out");
+    RR_REQUIRE(rr::is_device_ptr(lfac) && rr::is_device_ptr(j_out) && rr::is_device_ptr(k_out), RR_ERR_ARG,
+               "rr_ri_jk_factor_noalloc: lfac, j_out and k_out must be device memory");
     rr::check_prec(precision);
-    rr::PrecScope prec_scope(precision);
-    rr::Tensor& t = rr::get_slot(c, slot);
+    const rr::Tensor& t = rr::get_slot(c, slot);
     RR_REQUIRE(t.n == n, RR_ERR_DIM, "rr_ri_jk_factor: density dimension does not match the tensor");
-    double* jpart = nullptr;
-    int jsplit = 0;
-    const bool fused_j = j_out && !density;  // J of D = L R^T from the half-transformed factors
-    RR_REQUIRE(!fused_j || (lfac && r > 0 && k_out), RR_ERR_ARG,
-               "rr_ri_jk_factor: J without a density needs the factor and the exchange build");
-    if (j_out && density) {
-      const double* dD = rr::in_buf(c, rr::WS_P, density, n * n);
-      jsplit = rr::coulomb_real_prec(c, t, precision, dD, &jpart, t.symmetric);
+    // The half-transformed factors dominate the workspace: n_aux * n * roundup(r, 8) operands.
+    const size_t w_bytes = t.naux * n * rr::roundup(r, 8) * rr::prec_size(precision);
+    if (w_bytes + w_bytes / 4 + (size_t(8) << 20) > rr::kNoAllocScratchBytes) {
+      char msg[240];
+      snprintf(msg, sizeof(msg),
+               "rr_ri_jk_factor_noalloc: n_aux=%zu n=%zu r=%zu needs more than the %zu MiB scratch region "
+               "(the half-transformed factors alone take %zu MiB); use rr_ri_jk_factor",
+               t.naux, n, r, rr::kNoAllocScratchBytes >> 20, w_bytes >> 20);
+      throw rr::Error{RR_ERR_ARG, msg};
     }
-    rr::ExchangeOut ex;
-    if (k_out && r > 0) {
-      RR_REQUIRE(lfac, RR_ERR_ARG, "rr_ri_jk_factor: K requested without a factor");
-      const double* dL = rr::in_buf(c, rr::WS_CA, lfac, n * r);
-      const double* dR = rfac ? rr::in_buf(c, rr::WS_CB, rfac, n * r) : nullptr;
-      // Lt = L^T (r x n)
-      double* lt = static_cast<double*>(rr::wsbuf(c, rr::WS_T1, 2 * r * n * sizeof(double)));
-      hipLaunchKernelGGL((rr::k_pack<double>), dim3(rr::grid_for(r * n)), dim3(256), 0, c.stream, dL, r, 1, r, r, n, n, 1.0,
-                         lt);
-      if (dR)
-        hipLaunchKernelGGL((rr::k_pack<double>), dim3(rr::grid_for(r * n)), dim3(256), 0, c.stream, dR, r, 1, r, r, n, n,
-                           1.0, lt + r * n);
-      RR_HIP(hipGetLastError());
-      double* dfac = nullptr;
-      double* gamma = nullptr;
-      if (fused_j) {
-        const size_t rp = rr::roundup(r, 8);
-        const int nblk = dR ? 2 : 1;
-        dfac = static_cast<double*>(rr::wsbuf(c, rr::WS_DENS2, n * nblk * rp * sizeof(double)));
-        hipLaunchKernelGGL(rr::k_build_dfac, dim3(rr::grid_for(n * nblk * rp)), dim3(256), 0, c.stream,
-                           dR ? lt + r * n : lt, (const double*)nullptr, r, n, n, rp, nblk, 1.0, 0.0, dfac);
-        RR_HIP(hipGetLastError());
-        gamma = static_cast<double*>(rr::wsbuf(c, rr::WS_GAMMA, std::max<size_t>(1, t.naux) * sizeof(double)));
-      }
-      if (fused_j)
-        ex = rr::exchange_and_coulomb(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma, true,
-                                      t.symmetric, &jpart, &jsplit);
-      else
-        ex = rr::exchange_real_prec(c, t, precision, lt, dR ? lt + r * n : nullptr, r, n, dfac, gamma);
-    }
-    rr::OutBuf oj = rr::out_buf(c, rr::WS_OUT_J, j_out, n * n), ok = rr::out_buf(c, rr::WS_OUT_K, k_out, n * n);
-    if (k_out && r == 0) hipLaunchKernelGGL(rr::k_zero, dim3(rr::grid_for(n * n)), dim3(256), 0, c.stream, ok.dev, n * n);
-    rr::real_assemble(c, t, jpart, jsplit, (k_out && r > 0) ? &ex : nullptr, -0.5, nullptr, nullptr, oj.dev,
-                      (k_out && r > 0) ? ok.dev : nullptr, t.symmetric);
-    rr::finish_out(c, oj);
-    rr::finish_out(c, ok);
+    rr::ArenaScope arena(c, base, rr::kNoAllocScratchBytes);
+    rr::PhaseTimer::suppressed = true;
+    struct Restore {
+      ~Restore() { rr::PhaseTimer::suppressed = false; }
+    } restore;
+    rr::ri_jk_factor_body(c, slot, precision, n, nullptr, lfac, nullptr, r, j_out, k_out);
   });
 }
 

@@ -42,7 +42,7 @@ def summary(out):
 
 
 def test_bench_call_sequence_is_memory_safe(replay_exe):
-    rc, out, err = run(replay_exe, "main,batch,sigma,rel,rows")
+    rc, out, err = run(replay_exe, "main,batch,sigma,rel,rows,noalloc")
     s = summary(out)
     assert s["violations"] == 0, err[-3000:]
     assert s["calls"] == 0 and rc == 0, out[-3000:]

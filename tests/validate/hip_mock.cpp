@@ -180,6 +180,7 @@ void** __hipRegisterFatBinary(const void*) {
   return handle;
 }
 void __hipUnregisterFatBinary(void**) {}
+void __hipRegisterVar(void**, void*, char*, const char*, int, size_t, int, int) {}
 void __hipRegisterFunction(void**, const void*, char*, const char*, unsigned int, void*, void*, void*, void*, int*) {}
 hipError_t __hipPushCallConfiguration(dim3, dim3, size_t, hipStream_t) { return hipSuccess; }
 hipError_t __hipPopCallConfiguration(dim3*, dim3*, size_t*, hipStream_t*) { return hipSuccess; }
@@ -200,6 +201,24 @@ hipError_t hipMalloc(void** p, size_t size) {
   a.serial = ++st().serial;
   st().allocs[reinterpret_cast<uintptr_t>(m)] = a;
   *p = m;
+  return hipSuccess;
+}
+
+// Device variables: each symbol gets one allocation of kSymbolBytes (the stand-in does not know
+// the variable's size; the library's largest is its 160 MiB scratch region).
+hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+  constexpr size_t kSymbolBytes = size_t(256) << 20;
+  static std::map<const void*, void*> symbols;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = symbols.find(symbol);
+  if (it == symbols.end()) {
+    void* m = nullptr;
+    const hipError_t e = hipMalloc(&m, kSymbolBytes);
+    if (e != hipSuccess) return e;
+    it = symbols.emplace(symbol, m).first;
+  }
+  if (p) *p = it->second;
   return hipSuccess;
 }
 

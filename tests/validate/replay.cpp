@@ -5,7 +5,7 @@
 // caller array allocated at the extent bench.py gives it. No kernel runs; every launch, copy and
 // solver call is checked by the validation hooks. TEST INFRASTRUCTURE ONLY.
 //
-// Usage: replay [legs]   legs: any of "main,batch,sigma,rel,rows" (default: all, in bench order);
+// Usage: replay [legs]   legs: any of "main,batch,sigma,rel,rows,noalloc" (default: all, in that order);
 // "rows" replays the per-row leg as bench.py calls it; "plant" runs planted caller mistakes. Exit status: 0 when no violation and no
 // failed call, 1 otherwise. The number of violations is printed last.
 #include <hip/hip_runtime_api.h>
@@ -261,10 +261,25 @@ void leg_plant() {
   call("rr_tensor_release slot 1", rr_tensor_release(H, 1));
 }
 
+// The allocation-free build: two calls in a row, the second smaller, on the static scratch region.
+void leg_noalloc() {
+  std::printf("noalloc: rr_ri_jk_factor_noalloc, n_aux 800 / n 200 / r 40, then 300 / 24 / 5\n");
+  const size_t shapes[2][3] = {{800, 200, 40}, {300, 24, 5}};
+  for (const auto& sh : shapes) {
+    const size_t naux = sh[0], n = sh[1], r = sh[2];
+    call("rr_tensor_generate slot 8", rr_tensor_generate(H, 8, naux, n, 1.0, 20.0, SEED, ALL));
+    double *l = dev(n * r), *j = dev(n * n), *k = dev(n * n);
+    for (int prec : {RR_FP64, RR_FP32, RR_BF16})
+      call("rr_ri_jk_factor_noalloc", rr_ri_jk_factor_noalloc(H, 8, prec, n, l, r, j, k));
+    release(l), release(j), release(k);
+    call("rr_tensor_release slot 8", rr_tensor_release(H, 8));
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-  const std::string legs = argc > 1 ? argv[1] : "main,batch,sigma,rel,rows";
+  const std::string legs = argc > 1 ? argv[1] : "main,batch,sigma,rel,rows,noalloc";
   setvbuf(stdout, nullptr, _IOLBF, 0);
   if (rr_create(0, &H) != RR_OK) {
     std::fprintf(stderr, "rr_create failed\n");
@@ -276,6 +291,7 @@ int main(int argc, char** argv) {
   if (has("sigma")) leg_sigma();
   if (has("rel")) leg_relativistic();
   if (has("rows")) leg_rows();
+  if (has("noalloc")) leg_noalloc();
   if (has("plant")) leg_plant();
   rr_destroy(H);
   std::printf("replay: %ld checks, %d failed calls, %ld violations\n", rrv_checks(), failed_calls, rrv_violations());
