@@ -104,17 +104,17 @@ class DensityGuessNet:
         return p, "mlp"
 
 
-def run_scf(ctx, mol, guess=None, slot=0, lin_dep=1e-10, **scf_options):
+def run_scf(ctx, mol, guess=None, slot=0, lin_dep=1e-10, copies=None, **scf_options):
     """NON_REL RI SCF of a Molecule with the GPU Fock build (nonRelFockMatrix RI path).
 
     guess: None for the core-Hamiltonian starting density, or a trained DensityGuessNet (which
-    itself falls back to the core guess). Returns (ScfResult, ints, source) with ints the integral
+    itself falls back to the core guess). copies: the tensor copies to keep (default FP64 only). Returns (ScfResult, ints, source) with ints the integral
     dict of the molecule plus its orthogonaliser X, and source "core" or "mlp"."""
     ints = mol.integrals()
     s, h = ints["S"], ints["h"]
     x = ctx.canonical_orthogonalize(s, lin_dep)
     ints["X"] = x
-    ctx.upload(slot, ints["eri3"])
+    ctx.upload(slot, ints["eri3"]) if copies is None else ctx.upload(slot, ints["eri3"], copies)
     if guess is None:
         p0, source = _scf.core_guess(h, x, mol.n_electrons), "core"
     else:
