@@ -128,3 +128,73 @@ def kramers_pair_occ(nl, npairs, seed):
         partner /= np.linalg.norm(partner)
         cols.extend([v, partner])
     return np.stack(cols, axis=1)
+
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x):
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def device_tensor_rows(n, sigma, decay, seed, p0, p1):
+    """Rows [p0, p1) of the tensor Context.generate(slot, n_aux, n, sigma, decay, seed) builds on the
+    device, (p1 - p0, n, n) float64: value(P, mu, nu) = sigma z exp(-|mu - nu| / decay) with z the
+    Box-Muller normal of two splitmix64 draws keyed by (seed, P, min(mu, nu), max(mu, nu)). The rows
+    depend on P alone, so a tensor generated with fewer aux rows is the head of a larger one."""
+    iu = np.triu_indices(n)
+    a, b = iu[0].astype(np.uint64), iu[1].astype(np.uint64)
+    dec = sigma * np.exp(-(iu[1] - iu[0]).astype(np.float64) / float(decay))
+    out = np.empty((p1 - p0, n, n))
+    s = np.uint64(seed)
+    with np.errstate(over="ignore"):
+        for P in range(p0, p1):
+            key = _splitmix64(s ^ _splitmix64((np.uint64(P) * np.uint64(n) + a) * np.uint64(n) + b))
+            k2 = _splitmix64(key)
+            u1 = ((key >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * (1.0 / 9007199254740993.0)
+            u2 = (k2 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+            v = np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2) * dec
+            blk = out[P - p0]
+            blk[iu] = v
+            blk[iu[1], iu[0]] = v
+    return out
+
+
+def kramers_pair_ukb(nl, ns, npairs, seed):
+    """Kramers-paired occupied spinors in the four UKB flavor blocks [La; Lb; Sa; Sb] (sizes nl, nl, ns,
+    ns): each vector v comes with its time-reversed partner (-conj(v_Lb), conj(v_La), -conj(v_Sb),
+    conj(v_Sa)); columns orthonormal. Returns (2nl + 2ns) x 2npairs."""
+    g = rng(seed)
+    n = 2 * nl + 2 * ns
+
+    def partner(v):
+        la, lb, sa, sb = v[:nl], v[nl:2 * nl], v[2 * nl:2 * nl + ns], v[2 * nl + ns:]
+        return np.concatenate([-np.conj(lb), np.conj(la), -np.conj(sb), np.conj(sa)])
+
+    cols = []
+    a = g.standard_normal((n, npairs)) + 1j * g.standard_normal((n, npairs))
+    for k in range(npairs):
+        v = a[:, k].copy()
+        for _ in range(2):
+            for w in cols:
+                v -= w * (w.conj() @ v)
+        v /= np.linalg.norm(v)
+        t = partner(v)  # orthogonal to v and to every earlier pair (time reversal is antiunitary)
+        t /= np.linalg.norm(t)
+        cols.extend([v, t])
+    return np.stack(cols, axis=1)
+
+
+def time_reversal_ukb(nl, ns):
+    """Unitary part U of the UKB time-reversal operator Theta = U K: (La, Lb, Sa, Sb) -> (-Lb, La, -Sb, Sa).
+    A time-reversal symmetric operator satisfies G = U conj(G) U^H."""
+    n = 2 * nl + 2 * ns
+    u = np.zeros((n, n))
+    for off, m in ((0, nl), (2 * nl, ns)):
+        u[off:off + m, off + m:off + 2 * m] = -np.eye(m)
+        u[off + m:off + 2 * m, off:off + m] = np.eye(m)
+    return u
