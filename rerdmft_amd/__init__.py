@@ -65,6 +65,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rerdmft_amd.build`")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 with the same soname, and
+        # whichever is loaded first serves both. Load PyTorch's first (when it is installed) so that
+        # device tensors created after this library's context see the runtime PyTorch expects.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.rr_last_error.restype = ctypes.c_char_p
         _lib.rr_last_error.argtypes = [ctypes.c_void_p]

@@ -506,6 +506,7 @@ __global__ void k_cplx_asm(CplxAsm a) {
         ki += a.kpart[2 * (s * a.kstride + src) + 1];
       }
       if (!lower) ki = -ki;
+      if (r == c) ki = 0.0;  // a Hermitian K has a real diagonal (the product leaves rounding there)
     }
     double vr = a.kfac * kr, vi = a.kfac * ki;
     for (int b = 0; b < a.nblk; ++b) {
@@ -1874,21 +1875,37 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   double* y = static_cast<double*>(wsbuf(c, WS_T3, no * no * sizeof(double)));
   double* err = static_cast<double*>(wsbuf(c, WS_TMP8 + 100, sizeof(double)));
   dgemm_small(c, false, 'T', 'N', no, no, n, dA, no, dB, no, q, no);  // C_new^T C_old
-  for (int it = 0; it < 60; ++it) {
-    dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
-    if (it % 3 == 0) {
-      hipLaunchKernelGGL(k_orth_err, dim3(1), dim3(256), 0, c.stream, z, no, err);
-      double e = 0;
-      RR_HIP(hipMemcpyAsync(&e, err, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-      RR_HIP(hipStreamSynchronize(c.stream));
-      if (e < 1e-14) break;
-      RR_REQUIRE(std::isfinite(e), RR_ERR_SOLVER, "rr_ri_jk_delta: orbital alignment diverged");
+  // The polar factor is invariant to a positive scale; dividing by the Frobenius norm puts every
+  // singular value in (0, 1], where the Newton-Schulz step X (3 - X^T X) / 2 converges to it (the
+  // orbitals need not be orthonormal in the plain metric, e.g. C^T S C = I, so the singular values of
+  // C_new^T C_old are not bounded by 1 themselves).
+  const double qn = std::sqrt(fro2(c, q, no, 1, false, "align_scale"));
+  bool aligned = qn > 0.0 && std::isfinite(qn);
+  if (aligned) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.0 / qn, q, 0.0, q, q);
+    aligned = false;
+    for (int it = 0; it < 120 && !aligned; ++it) {
+      dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
+      if (it % 3 == 0) {
+        hipLaunchKernelGGL(k_orth_err, dim3(1), dim3(256), 0, c.stream, z, no, err);
+        double e = 0;
+        RR_HIP(hipMemcpyAsync(&e, err, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        RR_HIP(hipStreamSynchronize(c.stream));
+        RR_CONTROL("align_residual", e);
+        if (!std::isfinite(e)) break;
+        aligned = e < 1e-13;  // the step below then leaves rounding level
+      }
+      dgemm_small(c, false, 'N', 'N', no, no, no, q, no, z, no, y, no);  // X (X^T X)
+      hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.5, q, -0.5, y, q);
     }
-    dgemm_small(c, false, 'N', 'N', no, no, no, q, no, z, no, y, no);  // X (X^T X)
-    hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.5, q, -0.5, y, q);
   }
   double* cp = static_cast<double*>(wsbuf(c, WS_T4, n * no * sizeof(double)));
-  dgemm_small(c, false, 'N', 'N', n, no, no, dA, no, q, no, cp, no);  // C' = C_new Q
+  // C' = C_new Q; without an alignment (occupied spaces with a nearly orthogonal direction) C' = C_new:
+  // the difference identity below holds for any C', the alignment only keeps dC small.
+  if (aligned)
+    dgemm_small(c, false, 'N', 'N', n, no, no, dA, no, q, no, cp, no);
+  else
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * no)), dim3(256), 0, c.stream, n * no, 1.0, dA, 0.0, dA, cp);
   double* dc = static_cast<double*>(wsbuf(c, WS_COCC, 2 * n * no * sizeof(double)));
   double* m = dc + n * no;
   hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * no)), dim3(256), 0, c.stream, n * no, 1.0, cp, -1.0, dB, dc);   // dC
