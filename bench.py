@@ -71,7 +71,7 @@ def parse():
                    help="trial densities in the shared-tensor leg (0: skip); n = 800 (100 occ + 700 virt), n_aux = 3200")
     p.add_argument("--relativistic", type=int, default=1,
                    help="1: time the X2C (n=400, n_aux=2400) and 4c RKB (nL=300, nS=700, n_aux=4200) builds")
-    p.add_argument("--rows", type=int, default=0,
+    p.add_argument("--rows", type=int, default=1,
                    help="1: time the other entry points (RKB from a density, Cholesky builds, MO transforms, "
                         "functional-stage products, metric) at mid-size shapes")
     p.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -157,10 +157,11 @@ def main():
         ctx.ri_jk_factor(0, None, l_dev[k], precision=prec, j_out=J, k_out=K)
         return J - 0.5 * K  # two-electron part of F (h = 0)
 
-    def mixed_sequence():
+    def mixed_sequence(prec=None):
+        prec = dprec if prec is None else prec
         f = full_build(0)
         for k in range(1, N_DELTA + 1):
-            ctx.ri_jk_delta(0, c_dev[k], c_dev[k - 1], precision=dprec, dj_out=dJ, dk_out=dK)
+            ctx.ri_jk_delta(0, c_dev[k], c_dev[k - 1], precision=prec, dj_out=dJ, dk_out=dK)
             f = f + (dJ - 0.5 * dK)
         pre = f
         f = full_build(N_DELTA)
@@ -212,6 +213,36 @@ def main():
         fp64_rate = builds / t_fp64
 
     rate = builds * a.steps / t_mixed
+
+    # Every scheme of incremental builds: rate of the same sequence, and the accuracy of its
+    # accumulated Fock matrix at D_20 before the final FP64 build (|dE| = |1/2 tr(D (F - F_64))|).
+    schemes = {}
+    for pname, prec in (("bf16", rr.BF16), ("f32", rr.FP32)):
+        mixed_sequence(prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pre_s, _ = mixed_sequence(prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        schemes[pname] = {"builds_per_s": builds / dt,
+                          "dE_before_refinement": 0.5 * float(torch.sum(d20 * (pre_s - f_ref))),
+                          "max_dF_before_refinement": float(torch.max(torch.abs(pre_s - f_ref)))}
+    progress("incremental schemes timed")
+
+    # The reference's density-taking build recovers the occupied orbitals from D (FP64, n x n):
+    # time it on the device (rr_nonrel_fock_ri at D_20, phase "recovery").
+    eye = torch.eye(n, dtype=torch.float64, device=dev)
+    hz = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    fo = torch.empty_like(hz)
+    ctx.nonrel_fock_ri(0, hz, eye, eye, d_dev[N_DELTA], out=fo)
+    torch.cuda.synchronize()
+    ctx.timing(enable=True, reset=True)
+    for _ in range(3):
+        ctx.nonrel_fock_ri(0, hz, eye, eye, d_dev[N_DELTA], out=fo)
+    torch.cuda.synchronize()
+    rec_ph = ctx.timing(enable=False, reset=True)
+    gpu_recovery_ms = rec_ph["recovery"][0] / max(1, rec_ph["recovery"][1])
+    progress("device recovery timed")
 
     # Roofline of the dominant kernel of the mixed sequence: the (phase, precision) with the largest
     # device time, from HIP events recorded on the library stream around each launch.
@@ -302,6 +333,12 @@ def main():
                    "delta_precision": a.delta_precision},
         "fp64_builds_per_s": fp64_rate,
         "speedup_vs_fp64": (rate / fp64_rate) if fp64_rate else None,
+        "incremental_schemes": {k: {**v, "speedup_vs_fp64": (v["builds_per_s"] / fp64_rate) if fp64_rate else None}
+                                for k, v in schemes.items()},
+        "gpu_recovery_ms": gpu_recovery_ms,
+        "value_with_recovery": builds * a.steps / (t_mixed + 2 * a.steps * gpu_recovery_ms * 1e-3),
+        "fp64_builds_per_s_with_recovery": (1.0 / (1.0 / fp64_rate + gpu_recovery_ms * 1e-3)) if fp64_rate else None,
+        "scf_check": scf_check(rr, ctx),
         "dE_after_refinement": de_fin,
         "dF_after_refinement": df_fin,
         "dE_before_refinement": de_pre,
@@ -317,6 +354,26 @@ def main():
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def scf_check(rr, ctx):
+    """Mixed-precision SCF (incremental BF16 builds on FP64 full builds, FP64 final iterations) against
+    the all-FP64 SCF through nonRelFockMatrix's RI path: water dimer in the s-type model basis."""
+    from rerdmft_amd import guess, molecule, scf
+    w = molecule.WATER_GEOMETRY
+    mol = molecule.Molecule(np.concatenate([molecule.WATER_CHARGES] * 2),
+                            np.concatenate([w, w + np.array([0.0, 0.0, 5.6])]), molecule.WATER_BASIS,
+                            molecule.WATER_AUX)
+    tol = dict(diis_size=5, energy_tolerance=1e-11, density_tolerance=1e-8, max_iterations=80)
+    ref, ints, _ = guess.run_scf(ctx, mol, slot=12, copies=rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16, **tol)
+    full, delta = scf.gpu_builds(ctx, 12, rr.BF16)
+    r = scf.nonrel_scf_incremental(full, delta, ints["h"], ints["X"], ints["S"], mol.n_electrons, ints["e_nuc"],
+                                   **tol)
+    ctx.release(12)
+    return {"system": f"water dimer, s-type model basis (n={mol.n}, n_aux={mol.n_aux})",
+            "fp64_scf_builds": ref.iterations, "mixed_full_builds": r.full_builds, "mixed_bf16_builds": r.delta_builds,
+            "E_fp64": ref.total_energy, "dE_mixed_vs_fp64": r.total_energy - ref.total_energy,
+            "converged": bool(ref.converged and r.converged)}
 
 
 def batch_leg(torch, rr, ctx, nmol, reps=5):
@@ -415,6 +472,15 @@ def _phases(torch, ctx, fn):
     torch.cuda.synchronize()
     ph = ctx.timing(enable=False, reset=True)
     return {k: round(v[0], 3) for k, v in ph.items() if v[1]}
+
+
+def _phase_rates(ph, g1, rk):
+    """Achieved TFLOP/s of the half-transform and the rank-k update from their device times (ms)."""
+    out = {}
+    for key, fl in (("exchange_half", g1), ("exchange_rank_k", rk)):
+        if ph.get(key):
+            out[key] = round(fl / ph[key] / 1e9, 1)
+    return out
 
 
 def relativistic_cpu(x2c_shape, rkb_shape):
@@ -602,13 +668,13 @@ def relativistic_leg(torch, rr, ctx, reps=2, cpu=False):
     AO->MO transform to 800 x 800 per P; 4c RKB riFockTwoElectronDirect (nL = 300, nS = 700, one shared
     n_aux = 4200 with the LL block non-zero on its first 1800 rows, 100 Kramers pairs). The AO tensors
     are real, as in the reference's RI sources; tensors are generated in HBM, and densities, orbitals
-    and outputs are device arrays. "reference_gemm_TFLOP_per_s" divides the flops of the reference's
-    own GEMM calls for the build (real B times complex C: 4 flops per multiply-add, complex times
-    complex: 8; full Hermitian blocks, aux rows that are zero left out) by the build time:
-      X2C exchange  n_aux n^2 n_occ (2 spins x 4 + 3 blocks x 8) = 32 n_aux n^2 n_occ
-      X2C MO        n_aux (8 n^2 n_mo + 8 n_mo^2 2n)   (real B legs per spin, then one complex leg)
-      RKB           sum_P n_occ (4 sum_f fs_f^2 + 8 sum_{f<=u} fs_f fs_u) over the flavors f of
-                    (nL, nL, nS, nS) whose tensor row P is non-zero."""
+    and outputs are device arrays. "achieved_TFLOP_per_s" divides the flops the kernels perform by the
+    build time, and per phase by that phase's device time (real flops; a complex multiply-add is 8):
+      half-transform  2 n_aux_f fs_f^2 (2 n_occ) per flavor f (real B_P times the Re / Im rows of the
+                      occupied coefficients; only the aux rows where the flavor's tensor is non-zero)
+      rank-k update   8 n_occ sum_P m_P (m_P + 1) / 2, m_P = the rows of the flavors live at P
+                      (lower triangle of the Hermitian K over all flavor blocks at once)
+      X2C MO          n_aux (8 n^2 n_mo + 16 n_mo^2 n)   (real legs per spin, complex legs per spin)"""
     from rerdmft_amd import synthetic as syn
     dev = torch.device("cuda:0")
     cp = rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16
@@ -625,12 +691,15 @@ def relativistic_leg(torch, rr, ctx, reps=2, cpu=False):
     f = torch.empty_like(h)
     torch.cuda.synchronize()
     x2c = {"n_spatial": nl, "n_aux": naux, "n_occ": 2 * npairs}
-    flops = 32.0 * naux * nl * nl * (2 * npairs)
+    no = 2 * npairs
+    g1 = 2 * (2.0 * naux * nl * nl * 2 * no)
+    rk = 8.0 * no * naux * (2 * nl) * (2 * nl + 1) / 2
     ref = None
     for pname, prec in names:
         dt = _timed(torch, lambda: ctx.x2c_fock_ri(slot, h, eye, eye, p, precision=prec, out=f), reps)
-        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "reference_gemm_TFLOP_per_s": flops / dt / 1e12,
-             "phase_ms": _phases(torch, ctx, lambda: ctx.x2c_fock_ri(slot, h, eye, eye, p, precision=prec, out=f))}
+        ph = _phases(torch, ctx, lambda: ctx.x2c_fock_ri(slot, h, eye, eye, p, precision=prec, out=f))
+        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "achieved_TFLOP_per_s": (g1 + rk) / dt / 1e12,
+             "phase_ms": ph, "phase_TFLOP_per_s": _phase_rates(ph, g1, rk)}
         if ref is None:
             ref = f.clone()
         else:
@@ -640,13 +709,13 @@ def relativistic_leg(torch, rr, ctx, reps=2, cpu=False):
     g = torch.Generator(device=dev)
     g.manual_seed(20261203 + 22)
     u, _ = torch.linalg.qr(torch.randn((nt, nt), dtype=torch.complex128, device=dev, generator=g))
-    mo_flops = naux * (8.0 * nl * nl * nt + 8.0 * nt * nt * (2 * nl))
+    mo_flops = naux * (8.0 * nl * nl * nt + 16.0 * nt * nt * nl)
     for pname, prec, odt in (("f64", rr.FP64, torch.complex128), ("f32", rr.FP32, torch.complex64)):
         mo = torch.empty((naux, nt, nt), dtype=odt, device=dev)
         torch.cuda.synchronize()
         dt = _timed(torch, lambda: ctx.build_ri_mo_eri(slot, slot, nl, 0, u, precision=prec, out_precision=prec,
                                                        out=mo), 1)
-        x2c["mo_transform_" + pname] = {"ms": 1e3 * dt, "reference_gemm_TFLOP_per_s": mo_flops / dt / 1e12}
+        x2c["mo_transform_" + pname] = {"ms": 1e3 * dt, "achieved_TFLOP_per_s": mo_flops / dt / 1e12}
         del mo
         torch.cuda.empty_cache()
     ctx.release(slot)
@@ -674,17 +743,16 @@ def relativistic_leg(torch, rr, ctx, reps=2, cpu=False):
     gout = torch.empty((n, n), dtype=torch.complex128, device=dev)
     torch.cuda.synchronize()
     rkb = {"n_large": nl, "n_small": ns, "n_aux": naux, "n_aux_large_nonzero": nll, "n_occ": no}
-    def per_p(fs):
-        return no * (4.0 * sum(a * a for a in fs) + 8.0 * sum(fs[i] * fs[j] for i in range(len(fs))
-                                                              for j in range(i, len(fs))))
-    rflops = nll * per_p((nl, nl, ns, ns)) + (naux - nll) * per_p((ns, ns))
+    g1 = 2 * (2.0 * nll * nl * nl * 2 * no) + 2 * (2.0 * naux * ns * ns * 2 * no)
+    rk = 8.0 * no * (nll * (2 * nl + 2 * ns) * (2 * nl + 2 * ns + 1) / 2 + (naux - nll) * (2 * ns) * (2 * ns + 1) / 2)
     ref = None
     for pname, prec in names:
         dt = _timed(torch, lambda: ctx.ri_fock_two_electron_direct(sl, ss, nl, ns, dens, cq, occ, precision=prec,
                                                                      out=gout), reps)
-        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "reference_gemm_TFLOP_per_s": rflops / dt / 1e12,
-             "phase_ms": _phases(torch, ctx, lambda: ctx.ri_fock_two_electron_direct(sl, ss, nl, ns, dens, cq, occ,
-                                                                                      precision=prec, out=gout))}
+        ph = _phases(torch, ctx, lambda: ctx.ri_fock_two_electron_direct(sl, ss, nl, ns, dens, cq, occ,
+                                                                          precision=prec, out=gout))
+        r = {"ms_per_build": 1e3 * dt, "builds_per_s": 1.0 / dt, "achieved_TFLOP_per_s": (g1 + rk) / dt / 1e12,
+             "phase_ms": ph, "phase_TFLOP_per_s": _phase_rates(ph, g1, rk)}
         if ref is None:
             ref = gout.clone()
         else:
