@@ -70,6 +70,7 @@ struct Context {
   int lds_override = -1;  // -1 auto, -2 generic kernel only, >= 0 forced LDS variant
   bool autotune = true;
   int last_ntiles = 0;  // tiles per batch of the most recent GEMM launch
+  int num_cu = 256;     // compute units of the device (persistent kernels launch one workgroup each)
   std::map<std::vector<long long>, std::pair<int, int>> tuned;  // shape key -> (generic cfg, LDS variant)
   float last_ms[8] = {0};
   RkbChol rkb[kRkbSets];

@@ -30,6 +30,13 @@ __host__ __device__ inline float bf16_to_f32(bf16_t v) {
 }
 
 __host__ __device__ inline bf16_t f32_to_bf16(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // one v_cvt_pk_bf16_f32 (round to nearest even); the branchy software form below cost about
+  // twenty instructions per value in the GEMM epilogues
+  bf16_t r;
+  r.bits = __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+  return r;
+#else
   union {
     uint32_t u;
     float f;
@@ -45,6 +52,7 @@ __host__ __device__ inline bf16_t f32_to_bf16(float f) {
   bf16_t r;
   r.bits = static_cast<uint16_t>(u >> 16);
   return r;
+#endif
 }
 
 template <typename T>

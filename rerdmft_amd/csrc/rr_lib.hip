@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "rr_core.hpp"
+#include "rr_gemm_rows.hpp"
 
 namespace rr {
 // Named grow-only device workspace buffers.
@@ -579,6 +580,7 @@ struct LdsCfg {
   int bm, bn, wgm;
   int bf16_only;  // single-buffered, higher-occupancy variants built for bf16 operands only
   int fp64_only;  // eight-wave fp64 / fp32 direct-to-LDS variants (not bf16)
+  int rows = 0;   // row-merged persistent kernel (rr_gemm_rows.hpp) with NT = bn / 16, 4 stages
 };
 static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 0}, {64, 208, 4, 0},
                               {64, 224, 2, 0},  {256, 208, 4, 0}, {128, 224, 2, 0}, {128, 208, 4, 1},
@@ -600,7 +602,9 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               // bf16 through the eight-wave full-line kernel (two matrix-core steps per stage): 34-36
                               {256, 208, 8, 1}, {256, 224, 4, 1}, {128, 208, 8, 1},
                               // fp64 / fp32 eight-wave square tiles for the rank-k update: 37-38
-                              {128, 128, 4, 0, 1}, {192, 192, 4, 0, 1}};
+                              {128, 128, 4, 0, 1}, {192, 192, 4, 0, 1},
+                              // row-merged persistent bf16 half-transform, 128 x 400 / 416 / 448 tiles: 39-41
+                              {128, 400, 2, 1, 0, 1}, {128, 416, 2, 1, 0, 1}, {128, 448, 2, 1, 0, 1}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 // rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
@@ -660,6 +664,9 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 34: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 256, 208, 8, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 35: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 256, 224, 4, 2, 8>), dim3(grid), dim3(512), 0, s, g); return;
           case 36: hipLaunchKernelGGL((gemm_nt_dma64_kernel<TI, TO, 128, 208, 8, 3, 8>), dim3(grid), dim3(512), 0, s, g); return;
+          case 39: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 25, 4>), dim3(grid), dim3(512), 0, s, g); return;
+          case 40: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 26, 4>), dim3(grid), dim3(512), 0, s, g); return;
+          case 41: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 28, 4>), dim3(grid), dim3(512), 0, s, g); return;
           default:
             if (launch_dma_agpr(v, std::is_same<TO, double>::value, g, grid, s)) return;
             break;
@@ -753,6 +760,23 @@ void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
   RR_REQUIRE(!use_lds || sp.lds < kNumLds, RR_ERR_ARG, "bad LDS tile configuration");
   const TileCfg t = kTiles[sp.cfg];
   const int BM = use_lds ? kLds[sp.lds].bm : 32 * t.wm, BN = use_lds ? kLds[sp.lds].bn : 32 * t.wn;
+  if (use_lds && kLds[sp.lds].rows) {
+    // one tall product over the merged (batch, row) rows, persistent workgroups (one per CU)
+    RR_REQUIRE(sp.prec == P16 && rows_kernel_fits(g, BN / 16, 4, prec_size(sp.out)), RR_ERR_ARG,
+               "row-merged GEMM configuration does not fit this product");
+    g.tiles_m = (int)(((long long)g.M * g.nbatch + BM - 1) / BM);
+    g.tiles_n = (g.N + BN - 1) / BN;
+    g.ntiles = g.tiles_m * g.tiles_n;
+    if (g.kper <= 0) g.kper = g.K;
+    c.last_ntiles = g.ntiles;
+    g.zpage = c.zpage;
+    const int grid = std::max(1, std::min(g.ntiles, c.num_cu));
+    if (sp.out == P16) launch_lds<bf16_t, bf16_t>(sp.lds, g, grid, c.stream);
+    else if (sp.out == P64) launch_lds<bf16_t, double>(sp.lds, g, grid, c.stream);
+    else throw Error{RR_ERR_ARG, "unsupported GEMM precision combination"};
+    RR_HIP(hipGetLastError());
+    return;
+  }
   g.tiles_m = (g.M + BM - 1) / BM;
   g.tiles_n = (g.N + BN - 1) / BN;
   if (g.tri) {
@@ -810,7 +834,8 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
     if (c.lds_override >= 0 && !sp.cplx && g.kbatch == 0) {
       const LdsCfg& v = kLds[c.lds_override];
       const bool fp64_wide = sp.prec == P64 && (c.lds_override == 2 || c.lds_override == 5 || c.lds_override == 6);
-      const bool unavailable = (v.bf16_only && sp.prec != P16) || (v.fp64_only && sp.prec == P16);
+      const bool unavailable = (v.bf16_only && sp.prec != P16) || (v.fp64_only && sp.prec == P16) ||
+                               (v.rows && (sp.out != P16 || !rows_kernel_fits(g, v.bn / 16, 4, prec_size(sp.out))));
       if ((!g.tri || v.bm == v.bn) && !fp64_wide && !unavailable) sp.lds = c.lds_override;
     }
     if (c.lds_override == -2) sp.lds = -1;
@@ -845,6 +870,7 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
       if (sp.prec == P64 && (v == 2 || v == 5 || v == 6)) continue;  // these exceed the fp64 register budget
       if (kLds[v].bf16_only && sp.prec != P16) continue;
       if (kLds[v].fp64_only && sp.prec == P16) continue;
+      if (kLds[v].rows && (sp.out != P16 || !rows_kernel_fits(g, kLds[v].bn / 16, 4, prec_size(sp.out)))) continue;
       cand.push_back({1, v});
     }
   hipEvent_t e0, e1;
@@ -3293,6 +3319,11 @@ int rr_create(int device, rr_handle* out) {
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->ctx.stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return RR_ERR_HIP;
+  }
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+      h->ctx.num_cu = cus;
   }
   if (rr::register_scratch(device) != hipSuccess) {
     (void)hipStreamDestroy(h->ctx.stream);

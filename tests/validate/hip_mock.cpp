@@ -291,6 +291,11 @@ hipError_t hipGetDeviceCount(int* n) {
   if (n) *n = 1;
   return hipSuccess;
 }
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t attr, int) {
+  if (!v) return hipErrorInvalidValue;
+  *v = attr == hipDeviceAttributeMultiprocessorCount ? 256 : 0;  // an MI355X has 256 compute units
+  return hipSuccess;
+}
 hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipGetLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "hip mock error"; }
