@@ -45,10 +45,11 @@ int rr_create(int device, rr_handle* out);
 int rr_destroy(rr_handle h);
 const char* rr_last_error(rr_handle h);
 /* Select the GEMM tile configuration: -1 per-shape automatic choice; 0..7 a register-direct
- * configuration for every launch; 100..141 an LDS-staged configuration for the hot real GEMMs
+ * configuration for every launch; 100..142 an LDS-staged configuration for the hot real GEMMs
  * (100..106 for every operand type, 107..125 and 130..136 for bf16 and 126..129, 137..138 for
- * fp64 / fp32 operands only; 139..141 the row-merged persistent bf16 half-transform, used where the
- * product's shape fits it; other operand types and shapes keep their default).
+ * fp64 / fp32 operands only; 139..141 the row-merged persistent bf16 half-transform and 142 the
+ * persistent split-K bf16 rank-k update, each used where the product's shape fits it; other
+ * operand types and shapes keep their default).
  * Results do not depend on the choice (every element is summed in the same order). */
 int rr_set_tile_config(rr_handle h, int cfg);
 /* How the density-taking builds (rr_nonrel_fock_ri, rr_x2c_fock_ri, rr_rkb_fock_ri) find the

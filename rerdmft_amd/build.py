@@ -13,7 +13,7 @@ SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rr_lib.hip", "rr_sigma.hip",
                                                        "rr_batch.hip", "rr_mlp.hip", "rr_diis.hip", "rr_cas.hip", "rr_gemm_wide.hip")]
 # translation units whose matrix-core accumulators stay in the AGPR form (wide tiles)
 AGPR_FORM = {"rr_gemm_wide.hip"}
-HEADERS = [os.path.join(HERE, "csrc", f) for f in ("rr_device.hpp", "rr_gemm.hpp", "rr_core.hpp")] + [
+HEADERS = [os.path.join(HERE, "csrc", f) for f in ("rr_device.hpp", "rr_gemm.hpp", "rr_gemm_rows.hpp", "rr_core.hpp", "rr_validate.hpp")] + [
     os.path.join(ROOT, "include", "rerdmft_gpu.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -232,3 +232,197 @@ __global__ __launch_bounds__(512) void gemm_rows_bf16_kernel(GemmArgs g) {
 }
 
 }  // namespace rr
+
+namespace rr {
+
+// Eligibility of a split-K product for gemm_rankk_bf16_kernel (checked on the host).
+inline bool rankk_kernel_fits(const GemmArgs& g) {
+  if (g.M <= 0 || g.N <= 0 || g.nbatch != 1 || g.tri || g.kbatch != 0) return false;
+  if (g.mcut != 0 || g.ncut != 0 || g.nblk != 0 || g.mrow != 0 || g.ldcn != 1) return false;
+  if (g.beta != 0.0 || g.alpha != 1.0 || g.lda % 8 != 0 || g.ldb % 8 != 0 || g.K % 8 != 0) return false;
+  if (g.nsplit > 1 && g.kper % 32 != 0) return false;
+  if (static_cast<long long>(g.M) * g.ldcm + g.N >= (1ll << 40)) return false;
+  if (reinterpret_cast<uintptr_t>(g.A) % 16 != 0 || reinterpret_cast<uintptr_t>(g.B) % 16 != 0) return false;
+  return true;
+}
+
+// Split-K rank-k update C_z[m][n] = sum_{k in split z} A[m][k] B[n][k] (bf16 operands, fp32 matrix-core
+// sums, TO partial outputs): the exchange step K = W_L W_R^T over the flat (aux row, column) index.
+// Same building blocks as gemm_rows_bf16_kernel: one persistent workgroup of eight waves per CU,
+// 256 x 256 tiles (waves of 128 x 64: eight A and four B fragments for 32 matrix-core steps per
+// stage), direct-to-LDS stages, DMA / fragment reads / matrix-core work interleaved, swapped
+// operands for four consecutive output columns per lane. Work items (tile, split) are dealt so that
+// the workgroups of one XCD take whole splits: the A and B panels of a split are then read through
+// that XCD's L2 once for all its tiles. The k order per output equals the other NT kernels' (32-k
+// slabs in increasing k inside the split), so partial outputs are bitwise equal to theirs.
+template <typename TO, int NST>
+__global__ __launch_bounds__(512) void gemm_rankk_bf16_kernel(GemmArgs g) {
+  constexpr int BM = 256, BN = 256, BK = 32, ROWB = 64, NW = 8;
+  constexpr int TI = (BM + BN) / 16;
+  constexpr int STAGE = (BM + BN) * ROWB;
+  constexpr int CH = TI / NW;  // DMA slots per wave and stage
+  constexpr int WM = 8, WN = 4;
+  static_assert(TI % NW == 0 && NST >= 3 && CH * (NST - 2) <= 15, "stage shape");
+  static_assert(NST * STAGE <= 160 * 1024, "stage buffers exceed the LDS");
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[NST * STAGE];
+
+  const int ntiles = g.tiles_m * g.tiles_n;
+  const int items = ntiles * g.nsplit;
+  const int G = static_cast<int>(gridDim.x);
+  const int w = static_cast<int>(blockIdx.x);
+  const int lslot = (G % 8 == 0) ? (w % 8) * (G / 8) + w / 8 : w;  // XCD-major slot of this workgroup
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = lane & 15, grp = lane >> 4;
+  const int wm0 = 128 * (wave >> 2), wn0 = 64 * (wave & 3);
+  const bf16_t* A = static_cast<const bf16_t*>(g.A);
+  const bf16_t* B = static_cast<const bf16_t*>(g.B);
+  const unsigned lbase = __builtin_amdgcn_readfirstlane(lds_addr(lds));
+
+  int tm = 0, tn = 0, split = 0, kbeg = 0, kend = 0, nstage = 0;
+  auto decode = [&](int item) {
+    split = item / ntiles;
+    const int tile = item - split * ntiles;
+    tm = tile % g.tiles_m, tn = tile / g.tiles_m;
+    kbeg = split * g.kper;
+    kend = min(g.K, kbeg + g.kper);
+    nstage = (kend - kbeg + BK - 1) / BK;
+  };
+  const bf16_t* src[CH];
+  int kch[CH];
+  auto setup = [&]() {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int row = 16 * (wave + NW * j) + (lane >> 2);
+      const int trow = row < BM ? row : row - BM;
+      const int c = (lane & 3) ^ ((trow >> 1) & 3);
+      kch[j] = 8 * c;
+      if (row < BM) {
+        const int m = min(tm * BM + row, g.M - 1);
+        src[j] = A + static_cast<long long>(m) * g.lda + 8 * c + kbeg;
+      } else {
+        const int n = min(tn * BN + trow, g.N - 1);
+        src[j] = B + static_cast<long long>(n) * g.ldb + 8 * c + kbeg;
+      }
+    }
+  };
+  auto dma = [&](int st, int j) {
+    const unsigned sb = lbase + static_cast<unsigned>((st % NST) * STAGE);
+    const int koff = st * BK;
+    const void* gp = (kbeg + koff + kch[j] < kend) ? static_cast<const void*>(src[j] + koff) : g.zpage;
+    lds_dma16(gp, sb + static_cast<unsigned>((wave + NW * j) * 1024));
+  };
+  auto issue_first = [&]() {
+    for (int st = 0; st < NST - 1 && st < nstage; ++st) {
+#pragma unroll
+      for (int j = 0; j < CH; ++j) dma(st, j);
+    }
+  };
+  auto read_a = [&](const unsigned char* base, int i) {
+    const int row = wm0 + 16 * i + r;
+    return *reinterpret_cast<const bf8*>(base + row * ROWB + ((grp ^ ((row >> 1) & 3)) << 4));
+  };
+  auto read_b = [&](const unsigned char* base, int j) {
+    const int row = wn0 + 16 * j + r;
+    return *reinterpret_cast<const bf8*>(base + (BM + row) * ROWB + ((grp ^ ((row >> 1) & 3)) << 4));
+  };
+  f4 acc[WM][WN];
+  // A fragments are replaced in place by the next stage's right after their last use; B fragments
+  // (used by every row group) are double-buffered. Keeps the wave within 256 VGPRs without spills.
+  bf8 fa[WM], fb0[WN], fb1[WN];
+  auto body = [&](const bf8 (&bcur)[WN], bf8 (&bnxt)[WN], int ist, int rst) {
+    const unsigned char* nbase = lds + ((rst < 0 ? 0 : rst) % NST) * STAGE;
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bcur[j], fa[i], acc[i][j], 0, 0, 0);
+      if (rst >= 0) {
+        fa[i] = read_a(nbase, i);
+        if (i < WN) bnxt[i] = read_b(nbase, i);
+      }
+      if (ist >= 0 && i < CH) dma(ist, i);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  // this wave's stages issued after the one needed before the work of stage s (s+1 .. min(s+NST-2, nstage-1))
+  auto wait_for = [&](int needed) {
+    const int later = max(0, min(needed + NST - 3, nstage - 1) - needed);
+    wait_vmcnt(CH * later);
+  };
+
+  int round = 0;
+  int item = lslot;
+  if (item >= items) return;
+  decode(item);
+  setup();
+  issue_first();
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): kernel arguments in
+  while (true) {
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    wait_vmcnt(CH * min(NST - 2, nstage - 1));  // stage 0 in; also retires the previous item's stores
+    __builtin_amdgcn_s_barrier();
+#pragma unroll
+    for (int i = 0; i < WM; ++i) fa[i] = read_a(lds, i);
+#pragma unroll
+    for (int j = 0; j < WN; ++j) fb0[j] = read_b(lds, j);
+    int st = 0;
+    for (; st + NST < nstage; st += 2) {
+      wait_vm<CH * (NST - 3)>();
+      __builtin_amdgcn_s_barrier();
+      body(fb0, fb1, st + NST - 1, st + 1);
+      wait_vm<CH * (NST - 3)>();
+      __builtin_amdgcn_s_barrier();
+      body(fb1, fb0, st + NST, st + 2);
+    }
+    for (; st < nstage; st += 2) {
+      wait_for(st + 1);
+      __builtin_amdgcn_s_barrier();
+      body(fb0, fb1, st + NST - 1 < nstage ? st + NST - 1 : -1, st + 1 < nstage ? st + 1 : -1);
+      if (st + 1 < nstage) {
+        wait_for(st + 2);
+        __builtin_amdgcn_s_barrier();
+        body(fb1, fb0, st + NST < nstage ? st + NST : -1, st + 2 < nstage ? st + 2 : -1);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    const int otm = tm, otn = tn, osplit = split;
+    ++round;
+    const int next = round * G + lslot;
+    if (next < items) {
+      decode(next);
+      setup();
+      issue_first();
+    }
+    TO* C = static_cast<TO*>(g.C) + static_cast<long long>(osplit) * g.sCz;
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      const int m = otm * BM + wm0 + 16 * i + r;
+      if (m >= g.M) continue;
+#pragma unroll
+      for (int j = 0; j < WN; ++j) {
+        const int n0 = otn * BN + wn0 + 16 * j + 4 * grp;
+        if (n0 >= g.N) continue;
+        TO* p = C + static_cast<long long>(m) * g.ldcm + n0;
+        if constexpr (std::is_same<TO, double>::value) {
+          if (n0 + 3 < g.N && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+            *reinterpret_cast<double2*>(p) = double2{acc[i][j][0], acc[i][j][1]};
+            *reinterpret_cast<double2*>(p + 2) = double2{acc[i][j][2], acc[i][j][3]};
+            continue;
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (n0 + e < g.N) store_out<TO>(p + e, static_cast<double>(acc[i][j][e]));
+      }
+    }
+    if (next >= items) break;
+    item = next;
+  }
+}
+
+}  // namespace rr

@@ -580,7 +580,8 @@ struct LdsCfg {
   int bm, bn, wgm;
   int bf16_only;  // single-buffered, higher-occupancy variants built for bf16 operands only
   int fp64_only;  // eight-wave fp64 / fp32 direct-to-LDS variants (not bf16)
-  int rows = 0;   // row-merged persistent kernel (rr_gemm_rows.hpp) with NT = bn / 16, 4 stages
+  int rows = 0;   // persistent kernels of rr_gemm_rows.hpp: 1 row-merged half-transform (NT = bn / 16),
+                  // 2 split-K rank-k update
 };
 static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 0}, {64, 208, 4, 0},
                               {64, 224, 2, 0},  {256, 208, 4, 0}, {128, 224, 2, 0}, {128, 208, 4, 1},
@@ -604,7 +605,9 @@ static const LdsCfg kLds[] = {{128, 128, 2, 0}, {128, 208, 4, 0}, {256, 128, 2, 
                               // fp64 / fp32 eight-wave square tiles for the rank-k update: 37-38
                               {128, 128, 4, 0, 1}, {192, 192, 4, 0, 1},
                               // row-merged persistent bf16 half-transform, 128 x 400 / 416 / 448 tiles: 39-41
-                              {128, 400, 2, 1, 0, 1}, {128, 416, 2, 1, 0, 1}, {128, 448, 2, 1, 0, 1}};
+                              {128, 400, 2, 1, 0, 1}, {128, 416, 2, 1, 0, 1}, {128, 448, 2, 1, 0, 1},
+                              // persistent split-K bf16 rank-k update, 256 x 256 tiles: 42
+                              {256, 256, 2, 1, 0, 2}};
 constexpr int kNumLds = sizeof(kLds) / sizeof(kLds[0]);
 
 // rr_gemm_wide.hip: direct-to-LDS variants 17-19 and 23-24 compiled with AGPR accumulators.
@@ -667,6 +670,7 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 39: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 25, 4>), dim3(grid), dim3(512), 0, s, g); return;
           case 40: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 26, 4>), dim3(grid), dim3(512), 0, s, g); return;
           case 41: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 28, 4>), dim3(grid), dim3(512), 0, s, g); return;
+          case 42: hipLaunchKernelGGL((gemm_rankk_bf16_kernel<TO, 4>), dim3(grid), dim3(512), 0, s, g); return;
           default:
             if (launch_dma_agpr(v, std::is_same<TO, double>::value, g, grid, s)) return;
             break;
@@ -760,7 +764,23 @@ void run_gemm(Context& c, const GemmSpec& sp, GemmArgs g) {
   RR_REQUIRE(!use_lds || sp.lds < kNumLds, RR_ERR_ARG, "bad LDS tile configuration");
   const TileCfg t = kTiles[sp.cfg];
   const int BM = use_lds ? kLds[sp.lds].bm : 32 * t.wm, BN = use_lds ? kLds[sp.lds].bn : 32 * t.wn;
-  if (use_lds && kLds[sp.lds].rows) {
+  if (use_lds && kLds[sp.lds].rows == 2) {
+    RR_REQUIRE(sp.prec == P16 && sp.out == P64 && rankk_kernel_fits(g), RR_ERR_ARG,
+               "persistent rank-k configuration does not fit this product");
+    g.tiles_m = (g.M + BM - 1) / BM;
+    g.tiles_n = (g.N + BN - 1) / BN;
+    g.ntiles = g.tiles_m * g.tiles_n;
+    if (g.kper <= 0) g.kper = g.K;
+    c.last_ntiles = g.ntiles;
+    g.zpage = c.zpage;
+    const long long items = (long long)g.ntiles * g.nsplit;
+    RR_REQUIRE(items < (1ll << 31), RR_ERR_ARG, "GEMM grid too large");
+    const int grid = (int)std::max<long long>(1, std::min<long long>(items, c.num_cu));
+    launch_lds<bf16_t, double>(sp.lds, g, grid, c.stream);
+    RR_HIP(hipGetLastError());
+    return;
+  }
+  if (use_lds && kLds[sp.lds].rows == 1) {
     // one tall product over the merged (batch, row) rows, persistent workgroups (one per CU)
     RR_REQUIRE(sp.prec == P16 && rows_kernel_fits(g, BN / 16, 4, prec_size(sp.out)), RR_ERR_ARG,
                "row-merged GEMM configuration does not fit this product");
@@ -835,7 +855,8 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
       const LdsCfg& v = kLds[c.lds_override];
       const bool fp64_wide = sp.prec == P64 && (c.lds_override == 2 || c.lds_override == 5 || c.lds_override == 6);
       const bool unavailable = (v.bf16_only && sp.prec != P16) || (v.fp64_only && sp.prec == P16) ||
-                               (v.rows && (sp.out != P16 || !rows_kernel_fits(g, v.bn / 16, 4, prec_size(sp.out))));
+                               (v.rows == 1 && (sp.out != P16 || !rows_kernel_fits(g, v.bn / 16, 4, prec_size(sp.out)))) ||
+                               (v.rows == 2 && (sp.out != P64 || !rankk_kernel_fits(g)));
       if ((!g.tri || v.bm == v.bn) && !fp64_wide && !unavailable) sp.lds = c.lds_override;
     }
     if (c.lds_override == -2) sp.lds = -1;
@@ -870,7 +891,8 @@ void gemm_auto(Context& c, GemmSpec sp, const GemmArgs& g) {
       if (sp.prec == P64 && (v == 2 || v == 5 || v == 6)) continue;  // these exceed the fp64 register budget
       if (kLds[v].bf16_only && sp.prec != P16) continue;
       if (kLds[v].fp64_only && sp.prec == P16) continue;
-      if (kLds[v].rows && (sp.out != P16 || !rows_kernel_fits(g, kLds[v].bn / 16, 4, prec_size(sp.out)))) continue;
+      if (kLds[v].rows == 1 && (sp.out != P16 || !rows_kernel_fits(g, kLds[v].bn / 16, 4, prec_size(sp.out)))) continue;
+      if (kLds[v].rows == 2 && (sp.out != P64 || !rankk_kernel_fits(g))) continue;
       cand.push_back({1, v});
     }
   hipEvent_t e0, e1;
@@ -1210,6 +1232,9 @@ int gemm2_splits(size_t n, size_t kdim, int slab) {
   size_t ns = std::max<size_t>(1, 2048 / tiles);
   // at least 256 slabs of k per split: small products (the molecules of a batch) keep long k-loops
   ns = std::min<size_t>(ns, std::max<size_t>(1, kdim / (size_t)(slab * 256)));
+  // a multiple of 16 from 16 up: the persistent rank-k kernel deals 16 (256 x 256) tiles of an
+  // n <= 1024 product per split, so its work items fill whole rounds of the CUs
+  if (ns >= 16) ns -= ns % 16;
   return (int)std::min<size_t>(ns, 256);
 }
 

@@ -167,7 +167,7 @@ def test_empty_occupied_space(ctx):
 @pytest.mark.parametrize("naux,n,r", [(40, 37, 7), (120, 130, 29), (64, 200, 64)])
 def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
     """Every bf16 GEMM variant (register-staged, LDS-staged single/double buffered, multi-stage
-    direct-to-LDS, row-merged persistent) sums each output in the same k order: results must be
+    direct-to-LDS, persistent row-merged and split-K) sums each output in the same k order: results must be
     bitwise identical."""
     b = syn.ri_tensor(naux, n, 1.0, 10.0, 300 + n)
     g = syn.rng(301 + n)
@@ -178,7 +178,7 @@ def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
     ctx.upload(9, b, rr.COPY_BF16)
     outs = []
     try:
-        for cfg in [1, 2, 5, 100, 101] + list(range(102, 126)) + list(range(130, 137)) + [139, 140, 141]:
+        for cfg in [1, 2, 5, 100, 101] + list(range(102, 126)) + list(range(130, 137)) + [139, 140, 141, 142]:
             ctx.set_tile_config(cfg)
             dj, dk = ctx.ri_jk_delta(9, c_new, c_old, precision=rr.BF16)
             j, k = ctx.ri_jk_factor(9, None, l, precision=rr.BF16)
