@@ -79,7 +79,10 @@ int rr_tensor_release(rr_handle h, int slot);
 /* ---- NON_REL ------------------------------------------------------------------------------ */
 /* nonRelFockMatrix(h, RiNonRelEriSource, P): src/NON_REL/NonRelFockMatrixRi.cpp:27-100
  * (declared NonRelFockMatrixRi.h:39-40). F = h + J - K/2 with the reference's occupied-orbital
- * recovery from P (eigenvalues of X^T (S P S) X / 2 above 0.5). j_out / k_out may be NULL. */
+ * recovery from P (eigenvalues of X^T (S P S) X / 2 above 0.5). j_out / k_out may be NULL.
+ * On a slot holding Cholesky vectors (n_chol in place of n_aux) this is the factor form of
+ * nonRelFockMatrix(h, AoCholesky, P) (src/Utils/AoCholesky.cpp:108-149) for an SCF density:
+ * K = sum_L (B_L C)(B_L C)^T equals sum_L B_L P B_L when P = C occ C^T with C^T S C = I. */
 int rr_nonrel_fock_ri(rr_handle h, int slot, int precision, size_t nl, const double* h_core, const double* x_large,
                       size_t nx, const double* overlap, const double* density, double* fock, double* j_out,
                       double* k_out);
@@ -122,7 +125,9 @@ int rr_ri_jk_delta(rr_handle h, int slot, int precision, size_t n, size_t n_occ,
 
 /* ---- X2C ---------------------------------------------------------------------------------- */
 /* x2cFockMatrix(h, RiX2cEriSource, P): src/X2C_DHF/X2C_FockMatrixRi.cpp:35-162 (declared
- * X2C_FockMatrixRi.h:38-40). Complex 2nl x 2nl arrays; x_large is 2nl x nx. */
+ * X2C_FockMatrixRi.h:38-40). Complex 2nl x 2nl arrays; x_large is 2nl x nx. On a slot holding
+ * Cholesky vectors this is the factor form of x2cFockMatrix(h, AoCholesky, P)
+ * (src/Utils/AoCholesky.cpp:151-217) for an SCF density (P = C occ C^H with C^H S C = I). */
 int rr_x2c_fock_ri(rr_handle h, int slot, int precision, size_t nl, const double* h_x2c, const double* x_large,
                    size_t nx, const double* overlap, const double* density, double* fock);
 
@@ -161,10 +166,12 @@ int rr_ri_nonrel_spin_mo_eri_rotated(rr_handle h, int precision, size_t n_spatia
 int rr_ri_nonrel_spin_embedded_b(rr_handle h, size_t n_spatial, size_t n_aux, const double* b, double* out);
 
 /* ---- Cholesky vectors (slot uploaded from AoCholesky::vectors, (n_chol, n*n)) ---------------- */
-/* nonRelFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:108-149 (K = sum_L B_L P B_L). */
+/* nonRelFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:108-149 (K = sum_L B_L P B_L, any
+ * symmetric P). For an SCF density rr_nonrel_fock_ri on the same slot gives the factor form. */
 int rr_nonrel_fock_cholesky(rr_handle h, int slot, int precision, size_t n, const double* h_core, const double* p,
                             double* fock);
-/* x2cFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:151-217. */
+/* x2cFockMatrix(h, AoCholesky, P): src/Utils/AoCholesky.cpp:151-217 (density form, any Hermitian
+ * P). For an SCF density rr_x2c_fock_ri on the same slot gives the factor form. */
 int rr_x2c_fock_cholesky(rr_handle h, int slot, int precision, size_t n, const double* h_x2c, const double* p,
                          double* fock);
 /* aoCholeskyToMoSpinOrbital (spatial block C^T B_L C): src/Utils/AoCholesky.cpp:219-243. */

@@ -145,3 +145,25 @@ def test_cholesky_factor_form_equals_density_form(ctx, oracle, prec, tol):
     assert rel(ctx.nonrel_fock_cholesky(7, h, p), f_ref) < 1e-12
     j, k = ctx.ri_jk_factor(7, None, np.sqrt(2.0) * c, precision=prec)
     assert rel(h + j - 0.5 * k, f_ref) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(rr.FP64, 1e-11), (rr.FP32, 2e-5), (rr.BF16, 3e-2)])
+def test_cholesky_factor_form_from_a_density(ctx, oracle, prec, tol):
+    """Factor-form Cholesky builds from a density: the RI builds' occupied-set recovery
+    (NonRelFockMatrixRi.cpp:51-69, X2C_FockMatrixRi.cpp:65-95) run on a slot of Cholesky vectors give
+    K = sum_L (B_L C)(B_L C)^H with N_chol in place of n_aux, equal to the density form
+    K = sum_L B_L P B_L of AoCholesky.cpp:108-217 for an SCF density (P = C occ C^H, C^H S C = I)."""
+    from tests.test_gpu_relativistic import x2c_inputs
+    nl, nchol = 30, 110
+    b, s, x, p, h = x2c_inputs(nl, nchol, 6, 771)
+    ctx.upload(8, b, rr.COPY_FP64 | rr.COPY_FP32 | rr.COPY_BF16)
+    f_ref = oracle.x2c_fock_cholesky(b.reshape(nchol, -1), h, p)
+    assert rel(ctx.x2c_fock_cholesky(8, h, p), f_ref) < 1e-11
+    assert rel(ctx.x2c_fock_ri(8, h, x, s, p, precision=prec), f_ref) < tol
+    # NON_REL on the alpha block's spatial problem
+    s1, x1 = s[:nl, :nl].real.copy(), x[:nl, :nl].real.copy()
+    c = x1 @ syn.orthonormal_occ(nl, 5, 772)
+    pn = 2.0 * c @ c.T
+    hn = syn.symmetric(nl, 773)
+    g_ref = oracle.nonrel_fock_cholesky(b.reshape(nchol, -1), hn, pn)
+    assert rel(ctx.nonrel_fock_ri(8, hn, x1, s1, pn, precision=prec), g_ref) < tol

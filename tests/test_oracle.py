@@ -382,3 +382,28 @@ def test_planted_transpose_caught_by_cholesky_check(oracle, monkeypatch):
     j = np.einsum("Lab,Lqs,sq->ab", vec, vec, p)
     k = np.einsum("Lam,mn,Lnb->ab", vec, p, vec)
     assert np.max(np.abs(f - (h + j - 0.5 * k))) > 1e-6
+
+
+def test_factor_form_with_recovery_equals_cholesky_density_form(oracle):
+    """For an SCF density (P = C occ C^H with C^H S C = I) the RI builds' recovered-factor exchange
+    equals the density-form exchange sum_L B_L P B_L of AoCholesky.cpp:108-217 on the same vectors:
+    the identity behind running the factor-form builds on Cholesky slots."""
+    from tests.test_gpu_relativistic import x2c_inputs
+    nl, naux = 20, 60
+    b, s, x, p, h = x2c_inputs(nl, naux, 4, 401)
+    f_ri = oracle.x2c_fock_ri(b.reshape(naux, -1), x, s, h, p)
+    f_ch = oracle.x2c_fock_cholesky(b.reshape(naux, -1), h, p)
+    assert np.max(np.abs(f_ri - f_ch)) < 1e-12 * np.max(np.abs(f_ch))
+    n = 30
+    bb = syn.ri_tensor(naux, n, 0.5, 8.0, 402)
+    s1 = syn.spd_overlap(n, 403)
+    xx = syn.inverse_sqrt(s1)
+    c = xx @ syn.orthonormal_occ(n, 5, 404)
+    pp, hh = 2 * c @ c.T, syn.symmetric(n, 405)
+    g_ri = oracle.nonrel_fock_ri(bb.reshape(naux, -1), xx, s1, hh, pp)
+    g_ch = oracle.nonrel_fock_cholesky(bb.reshape(naux, -1), hh, pp)
+    assert np.max(np.abs(g_ri - g_ch)) < 1e-12 * np.max(np.abs(g_ch))
+    # a density that is not idempotent in the S metric (occupation 0.7) is outside the identity
+    p2 = 0.7 * c @ c.T
+    assert np.max(np.abs(oracle.nonrel_fock_ri(bb.reshape(naux, -1), xx, s1, hh, p2)
+                         - oracle.nonrel_fock_cholesky(bb.reshape(naux, -1), hh, p2))) > 1e-6
