@@ -404,7 +404,7 @@ def batch_leg(torch, rr, ctx, nmol, reps=5):
     for n in ns:
         t = torch.randn((4 * n, n, n), device=dev, dtype=torch.float64, generator=gen)
         tens.append(0.5 * (t + t.transpose(1, 2)))
-    batch = ctx.batch_create(tens, rr.COPY_FP64)
+    batch = ctx.batch_create(tens, rr.COPY_FP64 | rr.COPY_FP32)
     del tens
     ls = [torch.randn((n, max(1, n // 5)), device=dev, dtype=torch.float64, generator=gen) for n in ns]
     js = [torch.empty((n, n), device=dev, dtype=torch.float64) for n in ns]
@@ -422,11 +422,30 @@ def batch_leg(torch, rr, ctx, nmol, reps=5):
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - t0) / reps
     flops = sum(2.0 * (4 * n) * n * n * max(1, n // 5) + (4 * n) * n * (n + 1) * max(1, n // 5) for n in ns)
+    # FP32-MFMA mode (the bulk builds of a mixed SCF, FP64 for the final ones): the same batch from
+    # the FP32 copies, its own graph; errors against the FP64 results above
+    j64 = [j.clone() for j in js]
+    k64 = [k.clone() for k in ks]
+    js32 = [torch.empty_like(j) for j in js]
+    ks32 = [torch.empty_like(k) for k in ks]
+    ctx.batch_jk_factor(batch, ls, precision=rr.FP32, outs=(js32, ks32))
+    ctx.batch_jk_factor(batch, ls, precision=rr.FP32, outs=(js32, ks32))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.batch_jk_factor(batch, ls, precision=rr.FP32, outs=(js32, ks32))
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / reps
+    dj = max(float(torch.max(torch.abs(a - b)) / torch.max(torch.abs(b))) for a, b in zip(js32, j64))
+    dk = max(float(torch.max(torch.abs(a - b)) / torch.max(torch.abs(b))) for a, b in zip(ks32, k64))
     ctx.batch_destroy(batch)
     return {"molecules": nmol, "n_range": [min(ns), max(ns)], "dtype": "f64",
             "builds_per_s_graph": nmol / t_graph, "ms_per_batch_graph": 1e3 * t_graph,
             "ms_first_call_eager_plus_capture": 1e3 * t_first, "replayed": bool(rep),
-            "exchange_TFLOP_per_s": flops / t_graph / 1e12}
+            "exchange_TFLOP_per_s": flops / t_graph / 1e12,
+            "f32": {"builds_per_s_graph": nmol / t32, "ms_per_batch_graph": 1e3 * t32,
+                    "exchange_TFLOP_per_s": flops / t32 / 1e12,
+                    "max_rel_dJ_vs_f64": dj, "max_rel_dK_vs_f64": dk}}
 
 
 def sigma_leg(torch, rr, ctx, nd, reps=2, slot=14):

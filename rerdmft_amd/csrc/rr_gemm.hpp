@@ -855,6 +855,12 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma2_kernel(GemmArgs g) {
 // kernels of the operand type.
 // Row clamping, compile-time waits and the zero page for a partial stage as in gemm_nt_dma2_kernel.
 // ------------------------------------------------------------------------------------------------
+// 16-byte slot swizzle of a 128-byte staged row: the slot of chunk c is c ^ line_swizzle(row). A
+// 16-byte fragment read covers 16 consecutive rows in one pass of the 64 LDS banks: with (row mod 8)
+// alone, rows r and r + 8 land on the same banks (half of the LDS cycles were bank conflicts,
+// SQ_LDS_BANK_CONFLICT); flipping the low bit for the second group of eight rows separates them.
+__device__ inline int line_swizzle(int row) { return (row & 7) ^ ((row >> 3) & 1); }
+
 template <typename TI, typename TO, int BM, int BN, int WGM, int NST, int NW>
 __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
   constexpr bool B16 = std::is_same<TI, bf16_t>::value;  // two 32-k matrix-core steps per stage
@@ -917,7 +923,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
   for (int j = 0; j < MAXJ; ++j) {
     const int i = wave + NW * j;
     const int row = 8 * min(i, NDMA - 1) + (lane >> 3);
-    const int c = (lane & 7) ^ (row & 7);
+    const int c = (lane & 7) ^ line_swizzle(row);
     kch[j] = VE * c;
     if (row < BM) {
       const int m = min(bm0 + row, g.M - 1);
@@ -977,12 +983,12 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
 #pragma unroll
         for (int i = 0; i < WM; ++i) {
           const int lrow = wm0 + 16 * i + r;
-          MM::load(ga[i], reinterpret_cast<const TI*>(base + lrow * ROWB + (((4 * h + grp) ^ (lrow & 7)) << 4)));
+          MM::load(ga[i], reinterpret_cast<const TI*>(base + lrow * ROWB + (((4 * h + grp) ^ line_swizzle(lrow)) << 4)));
         }
 #pragma unroll
         for (int j = 0; j < WN; ++j) {
           const int lrow = BM + wn0 + 16 * j + r;
-          MM::load(gb[j], reinterpret_cast<const TI*>(base + lrow * ROWB + (((4 * h + grp) ^ (lrow & 7)) << 4)));
+          MM::load(gb[j], reinterpret_cast<const TI*>(base + lrow * ROWB + (((4 * h + grp) ^ line_swizzle(lrow)) << 4)));
         }
         MM::mma_tiles(acc, ga, gb);
       }
@@ -990,7 +996,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_nt_dma64_kernel(GemmArgs g) {
     }
     auto lfrag = [&](frag& f, int lrow) {
       const unsigned char* rp = base + lrow * ROWB;
-      const int x = lrow & 7;
+      const int x = line_swizzle(lrow);
       if constexpr (sizeof(TI) == 8) {
         const d2 a = *reinterpret_cast<const d2*>(rp + (((2 * grp) ^ x) << 4));
         const d2 b = *reinterpret_cast<const d2*>(rp + (((2 * grp + 1) ^ x) << 4));
