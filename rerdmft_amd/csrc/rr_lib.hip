@@ -1934,6 +1934,23 @@ __global__ void k_axpby(size_t count, double a, const double* x, double b, const
     out[i] = a * x[i] + b * y[i];
 }
 // out = 1.5 X - 0.5 X (X^T X) helper pieces: Y = 1.5*X - 0.5*Z
+// max_i sum_j |a_ij| of an m x m matrix into *out (single workgroup; one row per thread at a time)
+__global__ void k_max_row_abs_sum(const double* a, size_t m, double* out) {
+  __shared__ double sh[256];
+  double e = 0.0;
+  for (size_t i = threadIdx.x; i < m; i += blockDim.x) {
+    double r = 0.0;
+    for (size_t j = 0; j < m; ++j) r += fabs(a[i * m + j]);
+    e = fmax(e, r);
+  }
+  sh[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
 // ||X^T X - I||_max into *err (single workgroup)
 __global__ void k_orth_err(const double* xtx, size_t m, double* err) {
   __shared__ double sh[256];
@@ -1967,11 +1984,19 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   double* y = static_cast<double*>(wsbuf(c, WS_T3, no * no * sizeof(double)));
   double* err = static_cast<double*>(wsbuf(c, WS_TMP8 + 100, sizeof(double)));
   dgemm_small(c, false, 'T', 'N', no, no, n, dA, no, dB, no, q, no);  // C_new^T C_old
-  // The polar factor is invariant to a positive scale; dividing by the Frobenius norm puts every
-  // singular value in (0, 1], where the Newton-Schulz step X (3 - X^T X) / 2 converges to it (the
-  // orbitals need not be orthonormal in the plain metric, e.g. C^T S C = I, so the singular values of
-  // C_new^T C_old are not bounded by 1 themselves).
-  const double qn = std::sqrt(fro2(c, q, no, 1, false, "align_scale"));
+  // The polar factor is invariant to a positive scale; dividing by an upper bound of the largest
+  // singular value puts every singular value in (0, 1], where the Newton-Schulz step X (3 - X^T X) / 2
+  // converges to it (the orbitals need not be orthonormal in the plain metric, e.g. C^T S C = I, so
+  // the singular values of C_new^T C_old are not bounded by 1 themselves). The bound is
+  // sigma_max^2 <= ||X^T X||_inf (max row sum of |X^T X|): exact for an orthogonal X, whose alignment
+  // then needs no step at all (the Frobenius norm, sqrt(n_occ) times too large there, cost about ten).
+  dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
+  hipLaunchKernelGGL(k_max_row_abs_sum, dim3(1), dim3(256), 0, c.stream, z, no, err);
+  double bnd = 0.0;
+  RR_HIP(hipMemcpyAsync(&bnd, err, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  RR_HIP(hipStreamSynchronize(c.stream));
+  RR_CONTROL("align_scale", bnd);
+  const double qn = std::sqrt(bnd);
   bool aligned = qn > 0.0 && std::isfinite(qn);
   if (aligned) {
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.0 / qn, q, 0.0, q, q);
