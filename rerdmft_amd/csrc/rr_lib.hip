@@ -1934,8 +1934,9 @@ __global__ void k_axpby(size_t count, double a, const double* x, double b, const
     out[i] = a * x[i] + b * y[i];
 }
 // out = 1.5 X - 0.5 X (X^T X) helper pieces: Y = 1.5*X - 0.5*Z
-// max_i sum_j |a_ij| of an m x m matrix into *out (single workgroup; one row per thread at a time)
-__global__ void k_max_row_abs_sum(const double* a, size_t m, double* out) {
+// out[0] = b = max_i sum_j |a_ij| of an m x m matrix, out[1] = max_ij |a_ij / b - delta_ij| (single
+// workgroup of 256 threads)
+__global__ void k_align_scale(const double* a, size_t m, double* out) {
   __shared__ double sh[256];
   double e = 0.0;
   for (size_t i = threadIdx.x; i < m; i += blockDim.x) {
@@ -1949,7 +1950,17 @@ __global__ void k_max_row_abs_sum(const double* a, size_t m, double* out) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sh[0];
+  const double b = sh[0];
+  __syncthreads();
+  double d = 0.0;
+  for (size_t i = threadIdx.x; i < m * m; i += blockDim.x) d = fmax(d, fabs(a[i] / b - ((i / m) == (i % m) ? 1.0 : 0.0)));
+  sh[threadIdx.x] = d;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = b, out[1] = sh[0];
 }
 // ||X^T X - I||_max into *err (single workgroup)
 __global__ void k_orth_err(const double* xtx, size_t m, double* err) {
@@ -1982,7 +1993,7 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   double* q = static_cast<double*>(wsbuf(c, WS_T1, no * no * sizeof(double)));
   double* z = static_cast<double*>(wsbuf(c, WS_T2, no * no * sizeof(double)));
   double* y = static_cast<double*>(wsbuf(c, WS_T3, no * no * sizeof(double)));
-  double* err = static_cast<double*>(wsbuf(c, WS_TMP8 + 100, sizeof(double)));
+  double* err = static_cast<double*>(wsbuf(c, WS_TMP8 + 100, 2 * sizeof(double)));
   dgemm_small(c, false, 'T', 'N', no, no, n, dA, no, dB, no, q, no);  // C_new^T C_old
   // The polar factor is invariant to a positive scale; dividing by an upper bound of the largest
   // singular value puts every singular value in (0, 1], where the Newton-Schulz step X (3 - X^T X) / 2
@@ -1991,19 +2002,27 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   // sigma_max^2 <= ||X^T X||_inf (max row sum of |X^T X|): exact for an orthogonal X, whose alignment
   // then needs no step at all (the Frobenius norm, sqrt(n_occ) times too large there, cost about ten).
   dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
-  hipLaunchKernelGGL(k_max_row_abs_sum, dim3(1), dim3(256), 0, c.stream, z, no, err);
-  double bnd = 0.0;
-  RR_HIP(hipMemcpyAsync(&bnd, err, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  // bound and the orthogonality error of the scaled X in one pass and one read-back
+  hipLaunchKernelGGL(k_align_scale, dim3(1), dim3(256), 0, c.stream, z, no, err);
+  double be[2] = {0.0, 0.0};
+  RR_HIP(hipMemcpyAsync(be, err, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));
-  RR_CONTROL("align_scale", bnd);
+  RR_CONTROL("align_scale", be[0]);
+  const double bnd = be[0];
   const double qn = std::sqrt(bnd);
   bool aligned = qn > 0.0 && std::isfinite(qn);
   if (aligned) {
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.0 / qn, q, 0.0, q, q);
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.0 / bnd, z, 0.0, z, z);
     aligned = false;
     for (int it = 0; it < 120 && !aligned; ++it) {
-      dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
-      if (it % 3 == 0) {
+      if (it > 0) dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
+      if (it == 0) {
+        RR_CONTROL("align_residual", be[1]);
+        const double e = be[1];
+        if (!std::isfinite(e)) break;
+        aligned = e < 1e-13;  // an orthogonal overlap: one step below leaves rounding level
+      } else if (it % 3 == 0) {
         hipLaunchKernelGGL(k_orth_err, dim3(1), dim3(256), 0, c.stream, z, no, err);
         double e = 0;
         RR_HIP(hipMemcpyAsync(&e, err, sizeof(double), hipMemcpyDeviceToHost, c.stream));
