@@ -652,6 +652,39 @@ def rows_leg(torch, rr, ctx, reps=3):
     except Exception:
         pass
 
+    # a5 RKB: density form (K_fg = sum_L B^f P_fg B^g) and factor form (occupied set recovered in the
+    # RKB metric, K = sum_L W_L W_L^H) on the same Hermitian vectors and SCF-like density
+    try:
+        nl, nchol, nocc = 120, 1000, 100
+        n4 = 4 * nl
+        g = np.random.default_rng(SEED + 50)
+        large = g.standard_normal((nchol, nl, nl))
+        large = 0.5 * (large + large.transpose(0, 2, 1))
+        herm = lambda: (lambda a: 0.5 * (a + a.conj().transpose(0, 2, 1)))(  # noqa: E731
+            g.standard_normal((nchol, nl, nl)) + 1j * g.standard_normal((nchol, nl, nl)))
+        small0, small1 = herm(), herm()
+        cross = g.standard_normal((nchol, nl, nl)) + 0j
+        ctx.rkb_cholesky_upload(0, large, small0, small1, cross, cross)
+        del large, small0, small1, cross
+        a = g.standard_normal((n4, n4)) + 1j * g.standard_normal((n4, n4))
+        s = a @ a.conj().T / n4 + np.eye(n4)
+        w, v = np.linalg.eigh(s)
+        x = v / np.sqrt(w)
+        cq = x @ np.linalg.qr(g.standard_normal((n4, nocc)) + 1j * g.standard_normal((n4, nocc)))[0]
+        hr, pr, sr, xr = D(syn.hermitian(n4, SEED + 51)), D(cq @ cq.conj().T), D(s), D(x)
+        fr = torch.empty((n4, n4), dtype=torch.complex128, device=dev)
+        flop_d = 16 * 2 * 8.0 * nl ** 3 * nchol
+        row("a5_rkb_fock_cholesky", lambda: ctx.rkb_fock_cholesky(0, hr, pr, out=fr), flop=flop_d,
+            shape=f"nl={nl} (n_rkb={n4}) n_chol={nchol}")
+        row("a5_rkb_fock_cholesky_factor", lambda: ctx.rkb_fock_cholesky_factor(0, hr, sr, xr, pr, out=fr),
+            shape=f"nl={nl} (n_rkb={n4}) n_chol={nchol} n_occ={nocc} (recovered)")
+    except Exception as e:
+        out["a5_rkb_setup"] = {"error": f"{type(e).__name__}: {e}"[:300]}
+    try:
+        ctx.rkb_cholesky_release(0)
+    except Exception:
+        pass
+
     # a9 / a13 on a device MO tensor
     n, naux = 200, 800
     try:

@@ -291,6 +291,15 @@ int rr_rkb_cholesky_upload(rr_handle h, int set, size_t n_large, size_t n_chol, 
 int rr_rkb_cholesky_release(rr_handle h, int set);
 /* rkbFockMatrix(h_rkb, RkbCholesky, P): src/C4_DHF/RkbCholesky.cpp:352-448 (n = 4*nl). */
 int rr_rkb_fock_cholesky(rr_handle h, int set, size_t n, const double* h_rkb, const double* p, double* fock);
+/* rkbFockMatrix(h_rkb, RkbCholesky, P) in factor form: the exchange of src/C4_DHF/RkbCholesky.cpp:381-432
+ * (K_fg = sum_L B^f_L P_fg B^g_L) as K = sum_L W_L W_L^H with W^f_L = B^f_L F_f, F the occupied factor
+ * recovered from P in the RKB metric as the RI build does (UkbFockMatrixRi.cpp:185-211: eigenvectors
+ * of X^H (S P S) X above 1/2, weighted by sqrt(eigenvalue); s_full n x n, x_full n x nx complex).
+ * Equal to rr_rkb_fock_cholesky when P = F F^H (an SCF density); the Coulomb part, with its
+ * cross-flavor terms, is taken from P as there. Needs Hermitian large/small vectors (checked at
+ * upload; RR_ERR_ARG otherwise). */
+int rr_rkb_fock_cholesky_factor(rr_handle h, int set, size_t n, const double* h_rkb, const double* s_full,
+                                const double* x_full, size_t nx, const double* p, double* fock);
 /* rkbCholeskyToMo: src/C4_DHF/RkbCholesky.cpp:450-523. c_mo is (4 nl) x n_mo complex. With
  * n_negative = 0, out receives n_chol vectors of (n_mo x n_mo); otherwise the recompressed vectors
  * (RkbCholesky.cpp:244-283, eigenvalue cut threshold / n_chol) embedded in n_mo x n_mo. capacity =

@@ -35,7 +35,8 @@ EXPORTED = [
     "rr_hartree_exchange_sigma_hessian_vector_ri", "rr_hartree_exchange_sigma_joint_hessian_vector_ri",
     "rr_ri_three_center_tensor", "rr_transform_ri_three_center_ao_legs", "rr_canonical_orthogonalize",
     "rr_ri_mo_fock_rkb", "rr_ri_mo_fock_nonrel",
-    "rr_rkb_cholesky_upload", "rr_rkb_cholesky_release", "rr_rkb_fock_cholesky", "rr_rkb_cholesky_to_mo",
+    "rr_rkb_cholesky_upload", "rr_rkb_cholesky_release", "rr_rkb_fock_cholesky", "rr_rkb_fock_cholesky_factor",
+    "rr_rkb_cholesky_to_mo",
     "rr_batch_create", "rr_batch_destroy", "rr_batch_jk_factor",
     "rr_mlp_create", "rr_mlp_destroy", "rr_mlp_num_params", "rr_mlp_get_params", "rr_mlp_set_params",
     "rr_mlp_forward", "rr_mlp_loss_grad", "rr_mlp_train_step",
@@ -640,6 +641,18 @@ class Context:
         hh, pp = _c128(h_rkb), _c128(p)
         o, op = _out(out, (n, n), np.complex128)
         self._call("rr_rkb_fock_cholesky", ctypes.c_int(set_id), _sz(n), hh.ptr, pp.ptr, op)
+        return o
+
+    def rkb_fock_cholesky_factor(self, set_id, h_rkb, s_full, x_full, p, out=None):
+        """rkbFockMatrix(h_rkb, RkbCholesky, P) with the exchange in factor form (RkbCholesky.cpp:381-432
+        through the occupied set of P recovered in the RKB metric, UkbFockMatrixRi.cpp:185-211)."""
+        n = h_rkb.shape[0]
+        nx = x_full.shape[1]
+        hh, pp = _c128(h_rkb, n * n, "h_rkb"), _c128(p, n * n, "density")
+        ss, xx = _c128(s_full, n * n, "s_full"), _c128(x_full, n * nx, "x_full")
+        o, op = _out(out, (n, n), np.complex128)
+        self._call("rr_rkb_fock_cholesky_factor", ctypes.c_int(set_id), _sz(n), hh.ptr, ss.ptr, xx.ptr, _sz(nx), pp.ptr,
+                   op)
         return o
 
     def rkb_cholesky_to_mo(self, set_id, c_mo, n_chol, n_negative=0, threshold=1e-10):

@@ -48,6 +48,9 @@ struct RkbChol {
   size_t nl = 0, nchol = 0, ld = 0;
   double* nat = nullptr;
   double* tr = nullptr;
+  // max |B - B^H| over the large and the two small components, relative to max |B| (measured at
+  // upload): the factor-form exchange needs Hermitian flavor vectors
+  double herm_dev = 0.0;
 };
 
 // A captured HIP graph together with the shape it was recorded for and the workspace generation
@@ -188,6 +191,11 @@ void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, 
                  const double* B, size_t ldb, double* C, size_t ldc, double alpha = 1.0, double beta = 0.0);
 // Hermitian eigensolve in place (row j of a = eigenvector j), ascending eigenvalues in w.
 void eig_device(Context& c, bool cplx, size_t n, double* a, double* w);
+// Occupied set of a density: the eigenvectors of M = scale X^H (S P S) X with eigenvalue above 1/2 as
+// n_occ rows of length nx in *rows (device), with their weights in occ_host (eigenvalues, or ones for
+// the rows of a weighted factor F with F F^H = sum lambda v v^H when `weighted`). All pointers device.
+size_t recover_occupied(Context& c, bool cplx, size_t n, size_t nx, const double* dS, const double* dP, const double* dX,
+                        double scale, const double** rows, std::vector<double>& occ_host, bool weighted);
 Tensor& get_slot(Context& c, int slot);
 void check_prec(int prec);
 // FP64 MO tensors into device memory: buildRiMoEri (complex, (n_aux, nt^2)) and the embedded

@@ -19,7 +19,7 @@ namespace {
 enum RkbWs {
   WS_RK_STAGE = WS_SIGMA_BASE + 3000, WS_RK_MT, WS_RK_T, WS_RK_JPART, WS_RK_J, WS_RK_PST, WS_RK_TT, WS_RK_KPART,
   WS_RK_K, WS_RK_P, WS_RK_H, WS_RK_OUT, WS_RK_C, WS_RK_CT, WS_RK_CH, WS_RK_X, WS_RK_A, WS_RK_G, WS_RK_W, WS_RK_U,
-  WS_RK_AT, WS_RK_AP
+  WS_RK_AT, WS_RK_AP, WS_RK_S, WS_RK_XF, WS_RK_CT2, WS_RK_OCC, WS_RK_FP, WS_RK_WF, WS_RK_HERM
 };
 
 #define RR_GRID_LOOP(i, total) \
@@ -227,6 +227,58 @@ __global__ void k_rkb_transpose(const double* a, size_t rows, size_t lda, size_t
   }
 }
 
+// Hermiticity of the large and the two small components (tr holds them transposed): per workgroup,
+// out[2b] = max |nat - conj(tr)| and out[2b + 1] = max |nat| over its elements.
+__global__ __launch_bounds__(256) void k_rkb_herm(const double* nat, const double* tr, size_t count, double* out) {
+  __shared__ double sd[256], sm[256];
+  double d = 0.0, m = 0.0;
+  RR_GRID_LOOP(i, count) {
+    const double ar = nat[2 * i], ai = nat[2 * i + 1], br = tr[2 * i], bi = tr[2 * i + 1];
+    d = fmax(d, fmax(fabs(ar - br), fabs(ai + bi)));
+    m = fmax(m, fmax(fabs(ar), fabs(ai)));
+  }
+  sd[threadIdx.x] = d, sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + s]);
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[2 * blockIdx.x] = sd[0], out[2 * blockIdx.x + 1] = sm[0];
+}
+
+// Half-transform operand of the occupied factor F = C sqrt(occ), per flavor f:
+// fp[f][i][m] = C(f nl + m, i) sqrt(occ_i) from ct = C^T (n_occ x ldct complex); zero for i >= n_occ or
+// m >= nl (4 rp rows of ld).
+__global__ void k_rkb_fpack(const double* ct, size_t ldct, const double* occ, size_t n_occ, size_t nl, size_t rp,
+                            size_t ld, double* fp) {
+  RR_GRID_LOOP(i, 4 * rp * ld) {
+    const size_t row = i / ld, m = i % ld, f = row / rp, o = row % rp;
+    double re = 0.0, im = 0.0;
+    if (o < n_occ && m < nl) {
+      const double so = sqrt(occ[o]);
+      const size_t e = o * ldct + f * nl + m;
+      re = ct[2 * e] * so, im = ct[2 * e + 1] * so;
+    }
+    fp[2 * i] = re, fp[2 * i + 1] = im;
+  }
+}
+
+// K (N x N complex) from the lower-tile partial sums of W W^H: K[r][c] = sum_z part[z][r][c] for r >= c
+// and the conjugate of the mirrored element above the diagonal (the diagonal of a Hermitian K is real).
+__global__ void k_rkb_kherm(const double* part, int nparts, size_t N, double* K) {
+  const size_t nn = N * N;
+  RR_GRID_LOOP(i, nn) {
+    const size_t r = i / N, c = i % N;
+    const size_t e = r >= c ? i : c * N + r;
+    double sr = 0.0, si = 0.0;
+    for (int z = 0; z < nparts; ++z) sr += part[2 * (z * nn + e)], si += part[2 * (z * nn + e) + 1];
+    K[2 * i] = sr, K[2 * i + 1] = r > c ? si : (r == c ? 0.0 : -si);
+  }
+}
+
 RkbChol& get_set(Context& c, int set) {
   RR_REQUIRE(set >= 0 && set < kRkbSets, RR_ERR_ARG, "RKB Cholesky set out of range");
   RkbChol& r = c.rkb[set];
@@ -240,6 +292,32 @@ int splits_for(size_t m, size_t n, size_t ktot) {
   size_t ns = std::max<size_t>(1, 1024 / std::max<size_t>(1, tiles));
   ns = std::min(ns, std::max<size_t>(1, ktot / 64));
   return (int)ns;
+}
+
+// J of every component from the density (k_rkb_mt, traces, split partial sums over the vectors):
+// J[comp][a][b] (ld-padded rows, complex) as the reference accumulates t_L B_L (RkbCholesky.cpp:380-406).
+double* rkb_coulomb(Context& c, const RkbChol& r, const double* dp) {
+  const size_t nl = r.nl, ld = r.ld, per = nl * ld, nchol = r.nchol;
+  double* mt = static_cast<double*>(wsbuf(c, WS_RK_MT, 2 * 5 * per * sizeof(double)));
+  hipLaunchKernelGGL(k_rkb_mt, dim3(grid_for(5 * per)), dim3(256), 0, c.stream, dp, nl, ld, mt);
+  double* t = static_cast<double*>(wsbuf(c, WS_RK_T, std::max<size_t>(1, 2 * nchol) * sizeof(double)));
+  double* J = static_cast<double*>(wsbuf(c, WS_RK_J, 2 * 5 * per * sizeof(double)));
+  if (nchol) {
+    hipLaunchKernelGGL(k_rkb_trace, dim3((unsigned)std::min<size_t>(nchol, 65535)), dim3(256), 0, c.stream, r.nat, nchol,
+                       per, mt, t);
+    const size_t bx = std::min<size_t>((5 * per + 255) / 256, 4096);
+    size_t ns = std::max<size_t>(1, std::min<size_t>(nchol, 2048 / bx));
+    const size_t lper = (nchol + ns - 1) / ns;
+    ns = (nchol + lper - 1) / lper;
+    double* jp = static_cast<double*>(wsbuf(c, WS_RK_JPART, ns * 2 * 5 * per * sizeof(double)));
+    hipLaunchKernelGGL(k_rkb_j_part, dim3((unsigned)bx, (unsigned)ns), dim3(256), 0, c.stream, r.nat, nchol, per, t, lper,
+                       jp);
+    hipLaunchKernelGGL(k_rkb_j_reduce, dim3(grid_for(2 * 5 * per)), dim3(256), 0, c.stream, jp, (int)ns, 5 * per, J);
+  } else {
+    hipLaunchKernelGGL(k_zero3, dim3(grid_for(2 * 5 * per)), dim3(256), 0, c.stream, J, 2 * 5 * per);
+  }
+  RR_HIP(hipGetLastError());
+  return J;
 }
 
 }  // namespace
@@ -282,6 +360,19 @@ int rr_rkb_cholesky_upload(rr_handle h, int set, size_t n_large, size_t n_chol, 
         RR_HIP(hipStreamSynchronize(c.stream));  // the staging buffer is reused by the next chunk
       }
     }
+    // Hermiticity of the flavor components (large, small[0], small[1]), for the factor-form exchange
+    const size_t cnt = 3 * n_chol * per;
+    const int nb = rr::grid_for(cnt, 256, 1024);
+    double* hd = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_HERM, 2 * (size_t)nb * sizeof(double)));
+    hipLaunchKernelGGL(rr::k_rkb_herm, dim3(nb), dim3(256), 0, c.stream, r.nat, r.tr, cnt, hd);
+    RR_HIP(hipGetLastError());
+    std::vector<double> hh(2 * (size_t)nb);
+    RR_HIP(hipMemcpyAsync(hh.data(), hd, hh.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    RR_HIP(hipStreamSynchronize(c.stream));
+    double dmax = 0.0, mmax = 0.0;
+    for (int b = 0; b < nb; ++b) dmax = std::max(dmax, hh[2 * b]), mmax = std::max(mmax, hh[2 * b + 1]);
+    r.herm_dev = mmax > 0.0 ? dmax / mmax : 0.0;
+    RR_CONTROL("rkb_herm_dev", r.herm_dev);
   });
 }
 
@@ -294,27 +385,7 @@ int rr_rkb_fock_cholesky(rr_handle h, int set, size_t n, const double* h_rkb, co
     const size_t nl = r.nl, ld = r.ld, per = nl * ld, nchol = r.nchol, N = n;
     const double* dp = rr::in_buf(c, rr::WS_RK_P, p, 2 * N * N);
     const double* dh = rr::in_buf(c, rr::WS_RK_H, h_rkb, 2 * N * N);
-    // Coulomb: traces, then J = sum_L t_L B_L per component
-    double* mt = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_MT, 2 * 5 * per * sizeof(double)));
-    hipLaunchKernelGGL(rr::k_rkb_mt, dim3(rr::grid_for(5 * per)), dim3(256), 0, c.stream, dp, nl, ld, mt);
-    double* t = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_T, std::max<size_t>(1, 2 * nchol) * sizeof(double)));
-    double* J = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_J, 2 * 5 * per * sizeof(double)));
-    if (nchol) {
-      hipLaunchKernelGGL(rr::k_rkb_trace, dim3((unsigned)std::min<size_t>(nchol, 65535)), dim3(256), 0, c.stream, r.nat,
-                         nchol, per, mt, t);
-      const size_t bx = std::min<size_t>((5 * per + 255) / 256, 4096);
-      size_t ns = std::max<size_t>(1, std::min<size_t>(nchol, 2048 / bx));
-      const size_t lper = (nchol + ns - 1) / ns;
-      ns = (nchol + lper - 1) / lper;
-      double* jp = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_JPART, ns * 2 * 5 * per * sizeof(double)));
-      hipLaunchKernelGGL(rr::k_rkb_j_part, dim3((unsigned)bx, (unsigned)ns), dim3(256), 0, c.stream, r.nat, nchol, per, t,
-                         lper, jp);
-      hipLaunchKernelGGL(rr::k_rkb_j_reduce, dim3(rr::grid_for(2 * 5 * per)), dim3(256), 0, c.stream, jp, (int)ns, 5 * per,
-                         J);
-    } else {
-      hipLaunchKernelGGL(rr::k_zero3, dim3(rr::grid_for(2 * 5 * per)), dim3(256), 0, c.stream, J, 2 * 5 * per);
-    }
-    RR_HIP(hipGetLastError());
+    double* J = rr::rkb_coulomb(c, r, dp);  // Coulomb: traces, then J = sum_L t_L B_L per component
     // Exchange K_fg = sum_L B^f_L P_fg B^g_L
     double* K = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_K, 2 * N * N * sizeof(double)));
     hipLaunchKernelGGL(rr::k_zero3, dim3(rr::grid_for(2 * N * N)), dim3(256), 0, c.stream, K, 2 * N * N);
@@ -360,6 +431,87 @@ int rr_rkb_fock_cholesky(rr_handle h, int set, size_t n, const double* h_rkb, co
         }
       }
     }
+    rr::OutBuf of = rr::out_buf(c, rr::WS_RK_OUT, fock, 2 * N * N);
+    hipLaunchKernelGGL(rr::k_rkb_assemble, dim3(rr::grid_for(N * N)), dim3(256), 0, c.stream, dh, J, K, nl, ld, of.dev);
+    RR_HIP(hipGetLastError());
+    rr::finish_out(c, of);
+  });
+}
+
+int rr_rkb_fock_cholesky_factor(rr_handle h, int set, size_t n, const double* h_rkb, const double* s_full,
+                                const double* x_full, size_t nx, const double* p, double* fock) {
+  return guarded(h, [&](Context& c) {
+    rr::RkbChol& r = rr::get_set(c, set);
+    RR_REQUIRE(n % 4 == 0, RR_ERR_DIM, "rkbFockMatrix: RKB basis dimension must be a multiple of 4");
+    RR_REQUIRE(n / 4 == r.nl, RR_ERR_DIM, "rkbFockMatrix: RKB Cholesky dimension does not match H_RKB");
+    RR_REQUIRE(h_rkb && s_full && x_full && p && fock, RR_ERR_ARG, "rkbFockMatrix: null argument");
+    RR_REQUIRE(nx > 0 && nx <= n, RR_ERR_DIM, "rkbFockMatrix: x_full column count out of range");
+    RR_REQUIRE(r.herm_dev <= 1e-10, RR_ERR_ARG,
+               "rkbFockMatrix (Cholesky factor form): the large/small flavor vectors are not Hermitian");
+    const size_t nl = r.nl, ld = r.ld, per = nl * ld, nchol = r.nchol, N = n;
+    const double* dp = rr::in_buf(c, rr::WS_RK_P, p, 2 * N * N);
+    const double* dh = rr::in_buf(c, rr::WS_RK_H, h_rkb, 2 * N * N);
+    const double* dS = rr::in_buf(c, rr::WS_RK_S, s_full, 2 * N * N);
+    const double* dX = rr::in_buf(c, rr::WS_RK_XF, x_full, 2 * N * nx);
+    double* J = rr::rkb_coulomb(c, r, dp);
+    // occupied factor F = C sqrt(occ) with F F^H = P: eigenvectors of X^H (S P S) X above 1/2
+    const double* rows = nullptr;
+    std::vector<double> occ;
+    const size_t n_occ = rr::recover_occupied(c, true, N, nx, dS, dp, dX, 1.0, &rows, occ, true);
+    double* K = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_K, 2 * N * N * sizeof(double)));
+    if (n_occ && nchol) {
+      double* ct = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_CT2, 2 * n_occ * N * sizeof(double)));
+      rr::dgemm_small(c, true, 'N', 'T', n_occ, N, nx, rows, nx, dX, nx, ct, N);  // C^T
+      const double* docc = rr::to_device<double>(c, rr::WS_RK_OCC, occ.data(), n_occ);
+      const size_t rp = rr::roundup(n_occ, 8);
+      double* fp = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_FP, 2 * 4 * rp * ld * sizeof(double)));
+      hipLaunchKernelGGL(rr::k_rkb_fpack, dim3(rr::grid_for(4 * rp * ld)), dim3(256), 0, c.stream, ct, N, docc, n_occ, nl,
+                         rp, ld, fp);
+      RR_HIP(hipGetLastError());
+      // W[(f, a)][(L, i)] = (B^f_L F_f)[a][i] for a chunk of vectors (at most 4 GiB of W), then the
+      // lower tiles of W W^H into split partial sums; every chunk adds its own partial sums
+      const size_t wl = N * rp * 2 * sizeof(double);
+      const size_t lchunk = std::max<size_t>(1, std::min<size_t>(nchol, (size_t(4) << 30) / wl));
+      std::vector<int> nsplit;
+      std::vector<size_t> kper;
+      for (size_t l0 = 0; l0 < nchol; l0 += lchunk) {
+        const size_t ktot = std::min(lchunk, nchol - l0) * rp;
+        const int ns0 = rr::splits_for(N, N, ktot);
+        const size_t kp = rr::roundup((ktot + ns0 - 1) / ns0, 16);
+        kper.push_back(kp), nsplit.push_back((int)((ktot + kp - 1) / kp));
+      }
+      int nparts = 0;
+      for (int v : nsplit) nparts += v;
+      double* W = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_WF, lchunk * wl));
+      double* part = static_cast<double*>(rr::wsbuf(c, rr::WS_RK_KPART, (size_t)nparts * 2 * N * N * sizeof(double)));
+      int zoff = 0;
+      for (size_t l0 = 0, ci = 0; l0 < nchol; l0 += lchunk, ++ci) {
+        const size_t lc = std::min(lchunk, nchol - l0), ldw = lc * rp;
+        for (int f = 0; f < 4; ++f) {
+          rr::GemmArgs g;
+          g.A = r.nat + 2 * ((size_t)rr::flavor_comp(f) * nchol + l0) * per, g.lda = (long long)ld, g.sAb = (long long)per;
+          g.B = fp + 2 * (size_t)f * rp * ld, g.ldb = (long long)ld, g.sBb = 0;
+          g.C = W + 2 * (size_t)f * nl * ldw, g.ldcm = (long long)ldw, g.ldcn = 1, g.sCb = (long long)rp;
+          g.M = (int)nl, g.N = (int)rp, g.K = (int)ld, g.nbatch = (int)lc;
+          rr::GemmSpec sp;
+          sp.cplx = true, sp.cfg = 1;
+          rr::run_gemm(c, sp, g);
+        }
+        rr::GemmArgs g;
+        g.A = W, g.lda = (long long)ldw, g.B = W, g.ldb = (long long)ldw;
+        g.C = part + 2 * (size_t)zoff * N * N, g.ldcm = (long long)N, g.ldcn = 1, g.sCz = (long long)(N * N);
+        g.M = (int)N, g.N = (int)N, g.K = (int)ldw, g.kper = (int)kper[ci], g.nsplit = nsplit[ci];
+        g.tri = 1, g.conj = 2;
+        rr::GemmSpec sp;
+        sp.cplx = true, sp.cfg = 1;
+        rr::run_gemm(c, sp, g);
+        zoff += nsplit[ci];
+      }
+      hipLaunchKernelGGL(rr::k_rkb_kherm, dim3(rr::grid_for(N * N)), dim3(256), 0, c.stream, part, nparts, N, K);
+    } else {
+      hipLaunchKernelGGL(rr::k_zero3, dim3(rr::grid_for(2 * N * N)), dim3(256), 0, c.stream, K, 2 * N * N);
+    }
+    RR_HIP(hipGetLastError());
     rr::OutBuf of = rr::out_buf(c, rr::WS_RK_OUT, fock, 2 * N * N);
     hipLaunchKernelGGL(rr::k_rkb_assemble, dim3(rr::grid_for(N * N)), dim3(256), 0, c.stream, dh, J, K, nl, ld, of.dev);
     RR_HIP(hipGetLastError());

@@ -58,3 +58,56 @@ def test_rkb_cholesky_to_mo(ctx, nl, nchol, nmo, nn):
         ar = wr.reshape(-1, nmo, nmo)[:, nn:, nn:].reshape(wr.shape[0], m * m)
         assert rel(a.T @ a.conj(), ar.T @ ar.conj()) < 1e-10
         assert np.all(w.reshape(-1, nmo, nmo)[:, :nn, :] == 0) and np.all(w.reshape(-1, nmo, nmo)[:, :, :nn] == 0)
+
+
+def hermitian_case(seed, nl, nchol):
+    """RKB Cholesky vectors as the reference builds them: large real symmetric, the two small
+    (projected) components Hermitian; the cross components enter the Coulomb part only."""
+    g, large, s0, s1, cf, cb = rkb_case(seed, nl, nchol)
+    large = 0.5 * (large + large.transpose(0, 2, 1))
+    s0 = 0.5 * (s0 + s0.conj().transpose(0, 2, 1))
+    s1 = 0.5 * (s1 + s1.conj().transpose(0, 2, 1))
+    return g, large, s0, s1, cf, cb
+
+
+def rkb_density(g, n, n_occ, nx, occupations):
+    """An S-orthonormal occupied set in the RKB basis (S Hermitian positive definite, X its canonical
+    orthogonalizer with nx columns) and P = C diag(occ) C^H."""
+    a = g.standard_normal((n, n)) + 1j * g.standard_normal((n, n))
+    s = a @ a.conj().T / n + np.eye(n)
+    w, v = np.linalg.eigh(s)
+    x = (v / np.sqrt(w))[:, n - nx:]
+    q, _ = np.linalg.qr(g.standard_normal((nx, n_occ)) + 1j * g.standard_normal((nx, n_occ)))
+    c = x @ q
+    return s, x, (c * occupations) @ c.conj().T
+
+
+@pytest.mark.parametrize("nl,nchol,n_occ,frac", [(5, 11, 3, False), (13, 60, 10, False), (30, 150, 24, True),
+                                                 (12, 40, 0, False)])
+def test_rkb_fock_cholesky_factor_form(ctx, nl, nchol, n_occ, frac):
+    """Factor form (K = sum_L W_L W_L^H over the recovered occupied set) against the oracle's density form
+    for P = C diag(occ) C^H: idempotent, and occupations in (1/2, 1]."""
+    g, large, s0, s1, cf, cb = hermitian_case(300 + nl, nl, nchol)
+    n = 4 * nl
+    nx = n - 2
+    occ = g.uniform(0.55, 1.0, n_occ) if frac else np.ones(n_occ)
+    s, x, p = rkb_density(g, n, n_occ, nx, occ)
+    h = g.standard_normal((n, n)) + 1j * g.standard_normal((n, n))
+    h = 0.5 * (h + h.conj().T)
+    ctx.rkb_cholesky_upload(0, large, s0, s1, cf, cb)
+    f = ctx.rkb_fock_cholesky_factor(0, h, s, x, p)
+    fr = oracle.rkb_fock_cholesky(large, s0, s1, cf, cb, h, p)
+    assert rel(f, fr) < 1e-10
+    assert rel(f, ctx.rkb_fock_cholesky(0, h, p)) < 1e-10
+    ctx.rkb_cholesky_release(0)
+
+
+def test_rkb_fock_cholesky_factor_form_needs_hermitian_vectors(ctx):
+    g, large, s0, s1, cf, cb = rkb_case(7, 6, 9)  # small components not Hermitian
+    n = 24
+    s, x, p = rkb_density(g, n, 4, n, np.ones(4))
+    ctx.rkb_cholesky_upload(3, large, s0, s1, cf, cb)
+    import rerdmft_amd as rr
+    with pytest.raises(rr.GpuError, match="not Hermitian"):
+        ctx.rkb_fock_cholesky_factor(3, np.zeros((n, n), complex), s, x, p)
+    ctx.rkb_cholesky_release(3)
