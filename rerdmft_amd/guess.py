@@ -75,10 +75,16 @@ class DensityGuessNet:
         x = np.array([distances(p) for p in positions])
         return (x - self.x_mean) / self.x_std
 
+    def _pack(self, p):
+        return pack_triangle(p)
+
+    def _unpack(self, v):
+        return unpack_triangle(v, self.n)
+
     def fit(self, positions, densities, steps=10000, lr=1e-3, lr_final=1e-5):
         """Full-batch Adam on the GPU; the learning rate decays geometrically from lr to lr_final."""
         x = np.array([distances(p) for p in positions])
-        t = np.array([pack_triangle(d) for d in densities])
+        t = np.array([self._pack(d) for d in densities])
         self.x_mean = x.mean(axis=0)
         self.x_std = np.maximum(x.std(axis=0), 1e-6)
         self.t_mean = t.mean(axis=0)
@@ -94,13 +100,74 @@ class DensityGuessNet:
 
     def predict_raw(self, positions):
         y = self.ctx.mlp_forward(self.handle, self._features(positions), self.n_out)
-        return [unpack_triangle(self.t_mean + self.t_scale * row, self.n) for row in y]
+        return [self._unpack(self.t_mean + self.t_scale * row) for row in y]
 
     def initial_density(self, positions, h_core, overlap, x_large):
         """(P0, source): the purified prediction ("mlp") or the core-Hamiltonian guess ("core")."""
         p = purify(self.predict_raw([positions])[0], overlap, x_large, self.n_electrons)
         if p is None:
             return _scf.core_guess(h_core, x_large, self.n_electrons), "core"
+        return p, "mlp"
+
+
+def pack_hermitian(p):
+    """Real parameter vector of a Hermitian matrix: Re of the upper triangle, then Im above the diagonal."""
+    n = p.shape[0]
+    return np.concatenate([p.real[np.triu_indices(n)], p.imag[np.triu_indices(n, 1)]])
+
+
+def unpack_hermitian(v, n):
+    nt = n * (n + 1) // 2
+    re, im = np.zeros((n, n)), np.zeros((n, n))
+    re[np.triu_indices(n)] = v[:nt]
+    im[np.triu_indices(n, 1)] = v[nt:]
+    return (re + np.triu(re, 1).T) + 1j * (im - im.T)
+
+
+def purify_rkb(p, s_full, x_full, n_electrons):
+    """RKB density from an approximate one: the eigenvectors of X^H S P S X with the n_electrons
+    largest eigenvalues give spinors C = X V and P0 = C C^H; None when not finite or when the occupied
+    and the other eigenvalues are not separated by 1/2."""
+    if not np.all(np.isfinite(p)):
+        return None
+    m = x_full.conj().T @ (s_full @ p @ s_full) @ x_full
+    w, v = np.linalg.eigh(0.5 * (m + m.conj().T))
+    w, v = w[::-1], v[:, ::-1]
+    ne = n_electrons
+    if ne == 0 or w[ne - 1] <= 0.5 or (len(w) > ne and w[ne] >= 0.5):
+        return None
+    c = x_full @ v[:, :ne]
+    return c @ c.conj().T
+
+
+class DhfDensityNet(DensityGuessNet):
+    """Geometry -> converged 4c RKB density (complex Hermitian, n_rkb x n_rkb) for one molecule family;
+    the fallback is the 4c core guess (main.cpp:6117-6119). Targets are the n_rkb^2 real parameters of
+    pack_hermitian."""
+
+    def __init__(self, ctx, n_atoms, n_rkb, n_electrons, hidden=256, layers=3, seed=1):
+        self.ctx = ctx
+        self.n = n_rkb
+        self.n_electrons = n_electrons
+        self.n_in = n_atoms * (n_atoms - 1) // 2
+        self.n_out = n_rkb * n_rkb
+        self.handle = ctx.mlp_create(self.n_in, hidden, layers, self.n_out, seed)
+        self.x_mean = self.x_std = self.t_mean = None
+        self.t_scale = 1.0
+        self.losses = []
+
+    def _pack(self, p):
+        return pack_hermitian(p)
+
+    def _unpack(self, v):
+        return unpack_hermitian(v, self.n)
+
+    def initial_density(self, positions, h_rkb, s_full, x_full):
+        """(P0, source): the purified prediction ("mlp") or the 4c core guess ("core")."""
+        from . import dirac
+        p = purify_rkb(self.predict_raw([positions])[0], s_full, x_full, self.n_electrons)
+        if p is None:
+            return dirac.core_guess(h_rkb, x_full, self.n_electrons), "core"
         return p, "mlp"
 
 

@@ -323,6 +323,159 @@ def three_center(shells, aux_shells):
     return out
 
 
+def three_center_prim(shells, aux_shells):
+    """raw[P, mu, nu] = (mu nu | P) as three_center, for shells and auxiliary shells of one primitive
+    each, vectorized over the auxiliary shells of each angular momentum: per orbital shell pair and
+    auxiliary l, one Hermite-integral table R_tuv over all those auxiliary centres."""
+    if any(len(s.exps) != 1 for s in shells) or any(len(s.exps) != 1 for s in aux_shells):
+        raise ValueError("three_center_prim: every shell must have exactly one primitive")
+    off, n = _offsets(shells)
+    aoff, naux = _offsets(aux_shells)
+    out = np.zeros((naux, n, n))
+    groups = {}
+    for ic, sc in enumerate(aux_shells):
+        groups.setdefault(sc.l, []).append(ic)
+    gdata = []
+    for lc, idx in sorted(groups.items()):
+        g = np.array([aux_shells[i].exps[0] for i in idx])
+        cc = np.array([aux_shells[i].centre for i in idx])
+        w = np.array([aux_shells[i].coefs[0] for i in idx]) * _prim_norm(g, lc)
+        ec = hermite_e(lc, 0, g, np.zeros_like(g), np.zeros_like(g))  # [i, 0, t, ng]
+        comps = cart_components(lc)
+        # E^c with the (-1)^(tau+nu+phi) sign of the ket and the component's angular factor
+        e_c = np.zeros((len(comps), lc + 1, lc + 1, lc + 1, len(g)))
+        for z, (i, j, k) in enumerate(comps):
+            blk = ec[i, 0, :i + 1][:, None, None, :] * ec[j, 0, :j + 1][None, :, None, :] * ec[k, 0, :k + 1][None, None, :, :]
+            sg = (-1.0) ** (np.arange(i + 1)[:, None, None] + np.arange(j + 1)[None, :, None] + np.arange(k + 1)[None, None, :])
+            e_c[z, :i + 1, :j + 1, :k + 1] = blk * sg[..., None] * _angular_factor((i, j, k))
+        rows = np.array([aoff[i] for i in idx])[:, None] + np.arange(len(comps))[None, :]  # (ng, nc)
+        gdata.append((lc, g, cc, w, e_c, rows))
+    for ia, sa in enumerate(shells):
+        ca_list = cart_components(sa.l)
+        for ib in range(ia, len(shells)):
+            sb = shells[ib]
+            cb_list = cart_components(sb.l)
+            a, b = sa.exps[0], sb.exps[0]
+            p = a + b
+            pc = (a * sa.centre + b * sb.centre) / p
+            wab = sa.coefs[0] * _prim_norm(a, sa.l) * sb.coefs[0] * _prim_norm(b, sb.l)
+            ex = [hermite_e(sa.l, sb.l, a, b, sa.centre[k] - sb.centre[k]) for k in range(3)]
+            t1 = sa.l + sb.l + 1
+            e_ab = np.zeros((len(ca_list), len(cb_list), t1, t1, t1))
+            for x, ca in enumerate(ca_list):
+                for y, cb in enumerate(cb_list):
+                    e_ab[x, y, :ca[0] + cb[0] + 1, :ca[1] + cb[1] + 1, :ca[2] + cb[2] + 1] = (
+                        ex[0][ca[0], cb[0], :ca[0] + cb[0] + 1][:, None, None]
+                        * ex[1][ca[1], cb[1], :ca[1] + cb[1] + 1][None, :, None]
+                        * ex[2][ca[2], cb[2], :ca[2] + cb[2] + 1][None, None, :]
+                        * _angular_factor(ca) * _angular_factor(cb))
+            for lc, g, cc, w, e_c, rows in gdata:
+                lmax = t1 - 1 + lc
+                alpha = p * g / (p + g)
+                r = hermite_r(lmax, alpha, pc[None, :] - cc)  # (L+1, L+1, L+1, ng)
+                acc = np.zeros((t1, t1, t1, e_c.shape[0], len(g)))
+                for tau in range(lc + 1):
+                    for nu in range(lc + 1 - tau):
+                        for phi in range(lc + 1 - tau - nu):
+                            acc += e_c[:, tau, nu, phi, :][None, None, None] * r[tau:tau + t1, nu:nu + t1, phi:phi + t1, None, :]
+                pref = 2.0 * np.pi ** 2.5 / (p * g * np.sqrt(p + g)) * wab * w
+                val = np.einsum("xytuv,tuvcg->gcxy", e_ab, acc, optimize=True) * pref[:, None, None, None]
+                cols_a = off[ia] + np.arange(len(ca_list))
+                cols_b = off[ib] + np.arange(len(cb_list))
+                out[rows[:, :, None, None], cols_a[None, None, :, None], cols_b[None, None, None, :]] = val
+                out[rows[:, :, None, None], cols_b[None, None, None, :], cols_a[None, None, :, None]] = val
+    return out
+
+
+def _prim_group_data(aux_shells, sign):
+    """Per angular momentum, the single-primitive shells' exponents, centres, weights, Hermite
+    coefficients E_tuv of each component about its own centre (times (-1)^(t+u+v) when `sign`, the
+    ket side of a Coulomb integral, and the component's angular factor) and output rows."""
+    aoff, _ = _offsets(aux_shells)
+    groups = {}
+    for ic, sc in enumerate(aux_shells):
+        groups.setdefault(sc.l, []).append(ic)
+    out = []
+    for lc, idx in sorted(groups.items()):
+        g = np.array([aux_shells[i].exps[0] for i in idx])
+        cc = np.array([aux_shells[i].centre for i in idx])
+        w = np.array([aux_shells[i].coefs[0] for i in idx]) * _prim_norm(g, lc)
+        ec = hermite_e(lc, 0, g, np.zeros_like(g), np.zeros_like(g))
+        comps = cart_components(lc)
+        e_c = np.zeros((len(comps), lc + 1, lc + 1, lc + 1, len(g)))
+        for z, (i, j, k) in enumerate(comps):
+            blk = ec[i, 0, :i + 1][:, None, None, :] * ec[j, 0, :j + 1][None, :, None, :] * ec[k, 0, :k + 1][None, None, :, :]
+            if sign:
+                blk = blk * ((-1.0) ** (np.arange(i + 1)[:, None, None] + np.arange(j + 1)[None, :, None]
+                                        + np.arange(k + 1)[None, None, :]))[..., None]
+            e_c[z, :i + 1, :j + 1, :k + 1] = blk * _angular_factor((i, j, k))
+        rows = np.array([aoff[i] for i in idx])[:, None] + np.arange(len(comps))[None, :]
+        out.append((lc, idx, g, cc, w, e_c, rows))
+    return out
+
+
+def two_center_prim(aux_shells):
+    """(P|Q) as two_center, for single-primitive shells, vectorized over the Q shells of each angular
+    momentum."""
+    if any(len(s.exps) != 1 for s in aux_shells):
+        raise ValueError("two_center_prim: every shell must have exactly one primitive")
+    aoff, naux = _offsets(aux_shells)
+    out = np.zeros((naux, naux))
+    bra = _prim_group_data(aux_shells, False)
+    ket = _prim_group_data(aux_shells, True)
+    for la, idx_a, ga, ca, wa, ea, rows_a in bra:
+        t1 = la + 1
+        for ia in range(len(idx_a)):
+            a, ctr = ga[ia], ca[ia]
+            for lb, idx_b, gb, cb, wb, eb, rows_b in ket:
+                lmax = la + lb
+                alpha = a * gb / (a + gb)
+                r = hermite_r(lmax, alpha, ctr[None, :] - cb)
+                acc = np.zeros((t1, t1, t1, eb.shape[0], len(gb)))
+                for tau in range(lb + 1):
+                    for nu in range(lb + 1 - tau):
+                        for phi in range(lb + 1 - tau - nu):
+                            acc += eb[:, tau, nu, phi, :][None, None, None] * r[tau:tau + t1, nu:nu + t1, phi:phi + t1, None, :]
+                pref = 2.0 * np.pi ** 2.5 / (a * gb * np.sqrt(a + gb)) * wa[ia] * wb
+                val = np.einsum("xtuv,tuvcg->xgc", ea[:, :, :, :, ia], acc, optimize=True) * pref[None, :, None]
+                out[rows_a[ia][:, None, None], rows_b[None, :, :]] = val
+    return out
+
+
+def self_overlap_prim(shells):
+    """Diagonal of the overlap of single-primitive shells: coef^2 for every component (a unit
+    coefficient primitive shell's components are normalized, the angular factor makes each component
+    x^i y^j z^k carry the norm of x^l)."""
+    if any(len(s.exps) != 1 for s in shells):
+        raise ValueError("self_overlap_prim: every shell must have exactly one primitive")
+    return np.concatenate([np.full(s.size, float(s.coefs[0]) ** 2) for s in shells])
+
+
+def nabla(shells):
+    """D[k][mu, nu] = <mu | d/dx_k | nu> over the Cartesian functions of `shells` (unnormalized
+    contractions): d/dx x^j exp(-b x^2) = j x^(j-1) exp(-b x^2) - 2 b x^(j+1) exp(-b x^2) turns each
+    axis factor into overlap factors with the ket power lowered and raised by one."""
+    off, n = _offsets(shells)
+    d = np.zeros((3, n, n))
+    for ia, sa in enumerate(shells):
+        for ib, sb in enumerate(shells):
+            pr = _Pair(sa, sb)
+            e1 = [hermite_e(sa.l, sb.l + 1, pr.a, pr.b, np.full(pr.p.size, (sa.centre - sb.centre)[k])) for k in range(3)]
+            pref = pr.w * (np.pi / pr.p) ** 1.5
+            for x, ca in enumerate(cart_components(sa.l)):
+                for y, cb in enumerate(cart_components(sb.l)):
+                    f = _angular_factor(ca) * _angular_factor(cb)
+                    s0 = [e1[k][ca[k], cb[k], 0] for k in range(3)]
+                    for k in range(3):
+                        j = cb[k]
+                        dk = -2.0 * pr.b * e1[k][ca[k], j + 1, 0]
+                        if j > 0:
+                            dk = dk + j * e1[k][ca[k], j - 1, 0]
+                        others = np.prod([s0[m] for m in range(3) if m != k], axis=0)
+                        d[k, off[ia] + x, off[ib] + y] = f * float(np.sum(pref * dk * others))
+    return d
+
+
 def two_center(aux_shells):
     """(P|Q) over the Cartesian auxiliary functions (unnormalized contractions)."""
     aoff, naux = _offsets(aux_shells)
