@@ -238,16 +238,21 @@ namespace rr {
 // Eligibility of a split-K product for gemm_rankk_bf16_kernel (checked on the host).
 inline bool rankk_kernel_fits(const GemmArgs& g) {
   if (g.M <= 0 || g.N <= 0 || g.nbatch != 1 || g.tri || g.kbatch != 0) return false;
-  if (g.mcut != 0 || g.ncut != 0 || g.nblk != 0 || g.mrow != 0 || g.ldcn != 1) return false;
+  if (g.mcut != 0 || g.ncut != 0 || g.mrow != 0 || g.ldcn != 1) return false;
+  // column blocks: the four consecutive columns a lane stores lie in one block
+  if (g.nblk < 0 || g.nblk % 4 != 0) return false;
   if (g.beta != 0.0 || g.alpha != 1.0 || g.lda % 8 != 0 || g.ldb % 8 != 0 || g.K % 8 != 0) return false;
   if (g.nsplit > 1 && g.kper % 32 != 0) return false;
-  if (static_cast<long long>(g.M) * g.ldcm + g.N >= (1ll << 40)) return false;
+  const long long cend = g.nblk > 0 ? (g.N - 1) / g.nblk * g.sCnb + static_cast<long long>(g.M) * g.ldcm + g.nblk
+                                    : static_cast<long long>(g.M) * g.ldcm + g.N;
+  if (cend >= (1ll << 40)) return false;
   if (reinterpret_cast<uintptr_t>(g.A) % 16 != 0 || reinterpret_cast<uintptr_t>(g.B) % 16 != 0) return false;
   return true;
 }
 
 // Split-K rank-k update C_z[m][n] = sum_{k in split z} A[m][k] B[n][k] (bf16 operands, fp32 matrix-core
-// sums, TO partial outputs): the exchange step K = W_L W_R^T over the flat (aux row, column) index.
+// sums, TO partial outputs): the exchange step K = W_L W_R^T over the flat (aux row, column) index;
+// with column blocks (nblk) one launch writes the K of several stacked W_R that share W_L.
 // Same building blocks as gemm_rows_bf16_kernel: one persistent workgroup of eight waves per CU,
 // 256 x 256 tiles (waves of 128 x 64: eight A and four B fragments for 32 matrix-core steps per
 // stage), direct-to-LDS stages, DMA / fragment reads / matrix-core work interleaved, swapped
@@ -407,7 +412,12 @@ __global__ __launch_bounds__(512) void gemm_rankk_bf16_kernel(GemmArgs g) {
       for (int j = 0; j < WN; ++j) {
         const int n0 = otn * BN + wn0 + 16 * j + 4 * grp;
         if (n0 >= g.N) continue;
-        TO* p = C + static_cast<long long>(m) * g.ldcm + n0;
+        long long col = n0;
+        if (g.nblk > 0) {
+          const int blk = n0 / g.nblk;
+          col += static_cast<long long>(blk) * (g.sCnb - g.nblk);
+        }
+        TO* p = C + static_cast<long long>(m) * g.ldcm + col;
         if constexpr (std::is_same<TO, double>::value) {
           if (n0 + 3 < g.N && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
             *reinterpret_cast<double2*>(p) = double2{acc[i][j][0], acc[i][j][1]};

@@ -1409,6 +1409,16 @@ __global__ void k_gamma_w_reduce(const double* part, int nsplit, size_t kdim, si
   k_gamma_w_reduce_body(part, nsplit, kdim, rp, naux, gamma, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
+// Rows nu per partial sum of the gamma kernels (about 4096 workgroups per launch); a function of the
+// shape and the W type only, so every path that forms gamma from W sums each value in the same order.
+template <typename TW>
+size_t gamma_rows(size_t kdim, size_t rp, size_t n, size_t naux) {
+  size_t gx;
+  if constexpr (std::is_same<TW, double>::value) gx = (kdim / 8 + 255) / 256;
+  else gx = ((naux + 3) / 4 * (rp / 8) + 255) / 256;
+  return std::max<size_t>(8, (n * gx + 4095) / 4096);
+}
+
 // gamma[P] = sum_(nu,i) W[nu][(P,i)] dfac[nu][i] (+ W2 with dfac columns [rp, 2rp)) on the stream.
 template <typename TW>
 void gamma_from_w(Context& c, const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n, size_t naux,
@@ -1417,7 +1427,7 @@ void gamma_from_w(Context& c, const TW* W, const TW* W2, size_t kdim, size_t rp,
     // fp64 W: one 64-byte run per thread and row is already one full vector load per lane group;
     // the per-thread layout of k_gamma_w_partial streams it faster (measured)
     const size_t ng = kdim / 8, gx = (ng + 255) / 256;
-    const size_t rows = std::max<size_t>(8, (n * gx + 4095) / 4096);
+    const size_t rows = gamma_rows<TW>(kdim, rp, n, naux);
     const int nsplit = (int)((n + rows - 1) / rows);
     double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * ng * sizeof(double)));
     hipLaunchKernelGGL((k_gamma_w_partial<TW>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W, W2, kdim,
@@ -1429,12 +1439,92 @@ void gamma_from_w(Context& c, const TW* W, const TW* W2, size_t kdim, size_t rp,
   constexpr int NPP = 4;
   const size_t threads = ((naux + NPP - 1) / NPP) * (rp / 8);
   const size_t gx = (threads + 255) / 256;
-  const size_t rows = std::max<size_t>(8, (n * gx + 4095) / 4096);  // ~4096 workgroups
+  const size_t rows = gamma_rows<TW>(kdim, rp, n, naux);
   const int nsplit = (int)((n + rows - 1) / rows);
   double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nsplit * (kdim / 8) * sizeof(double)));
   hipLaunchKernelGGL((k_gamma_w_blocks<TW, NPP>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W, W2,
                      kdim, rp, n, naux, dfac, lddf, rows, part);
   hipLaunchKernelGGL(k_gamma_w_reduce, dim3(gamma_reduce_grid(naux)), dim3(256), 0, c.stream, part, nsplit, kdim, rp, naux, gamma);
+  RR_HIP(hipGetLastError());
+}
+
+// gamma partials of nd <= ND densities that share one W (no W2), density k weighted by
+// dfac[k][nu][i] (rows of length rp, densities sdf apart): the thread layout of k_gamma_w_blocks, with
+// the NPP x 8 values of W a thread owns in row nu loaded once for every density. Each value is summed
+// in the single-density kernels' order (rows nu of the slice, then the 8 columns), so the partials
+// equal theirs bit for bit. part[k] starts spart elements after part[k - 1].
+template <typename TW, int NPP, int ND>
+__global__ __launch_bounds__(256) void k_gamma_w_multi(const TW* W, size_t kdim, size_t rp, size_t n, size_t naux,
+                                                       const double* dfac, size_t sdf, int nd, size_t rows,
+                                                       double* part, size_t spart) {
+  const size_t g8n = rp / 8, nblk = (naux + NPP - 1) / NPP;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= nblk * g8n) return;
+  const size_t q = t % g8n, i0 = 8 * q, P0 = (t / g8n) * NPP;
+  const size_t nu0 = blockIdx.y * rows, nu1 = std::min(n, nu0 + rows);
+  double acc[ND][NPP];
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+#pragma unroll
+    for (int p = 0; p < NPP; ++p) acc[k][p] = 0.0;
+  for (size_t nu = nu0; nu < nu1; ++nu) {
+    double w[NPP][8];
+#pragma unroll
+    for (int p = 0; p < NPP; ++p) {
+      if (P0 + p < naux) {
+        load8<TW>(W + nu * kdim + (P0 + p) * rp + i0, w[p]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[p][e] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      if (k < nd) {
+        double d[8];
+        const double* df = dfac + k * sdf + nu * rp + i0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Vec16<double>::load(df + 2 * v, d + 2 * v);
+#pragma unroll
+        for (int p = 0; p < NPP; ++p)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) acc[k][p] += w[p][e] * d[e];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+    if (k < nd) {
+#pragma unroll
+      for (int p = 0; p < NPP; ++p)
+        if (P0 + p < naux) part[k * spart + blockIdx.y * (kdim / 8) + (P0 + p) * g8n + q] = acc[k][p];
+    }
+}
+
+// gamma of density blockIdx.y from its partials (k_gamma_w_reduce on part + y spart, gamma + y naux)
+__global__ void k_gamma_w_reduce_multi(const double* part, size_t spart, int nsplit, size_t kdim, size_t rp, size_t naux,
+                                       double* gamma) {
+  k_gamma_w_reduce_body(part + blockIdx.y * spart, nsplit, kdim, rp, naux, gamma + blockIdx.y * naux, blockIdx.x, 0,
+                        gridDim.x);
+}
+
+// gamma_k[P] = sum_(nu,i) W[nu][(P,i)] dfac[k][nu][i] for nd <= 8 densities in one pass over W (the
+// value gamma_from_w gives for each density alone, bit for bit).
+template <typename TW>
+void gamma_from_w_multi(Context& c, const TW* W, size_t kdim, size_t rp, size_t n, size_t naux, const double* dfac,
+                        int nd, double* gamma) {
+  constexpr int NPP = 4, ND = 8;
+  RR_REQUIRE(nd >= 1 && nd <= ND, RR_ERR_INTERNAL, "gamma_from_w_multi: density count");
+  const size_t threads = ((naux + NPP - 1) / NPP) * (rp / 8);
+  const size_t gx = (threads + 255) / 256;
+  const size_t rows = gamma_rows<TW>(kdim, rp, n, naux);
+  const int nsplit = (int)((n + rows - 1) / rows);
+  const size_t spart = (size_t)nsplit * (kdim / 8);
+  double* part = static_cast<double*>(wsbuf(c, WS_GPART, (size_t)nd * spart * sizeof(double)));
+  hipLaunchKernelGGL((k_gamma_w_multi<TW, NPP, ND>), dim3((unsigned)gx, (unsigned)nsplit), dim3(256), 0, c.stream, W,
+                     kdim, rp, n, naux, dfac, n * rp, nd, rows, part, spart);
+  hipLaunchKernelGGL(k_gamma_w_reduce_multi, dim3(gamma_reduce_grid(naux), (unsigned)nd), dim3(256), 0, c.stream, part,
+                     spart, nsplit, kdim, rp, naux, gamma);
   RR_HIP(hipGetLastError());
 }
 
@@ -2083,43 +2173,86 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
 // ================================================================================================
 // Many densities sharing the tensor and the left factor, D_k = L R_k^T (the trial densities of the
 // orbital-Hessian sigma products: L the occupied orbitals, R_k = C_virt T_k^T). W_L = B L is built
-// once; the R_k are half-transformed G at a time in one launch (column blocks of the output); then
-// per density gamma_k = <W_L, R_k> and K_k = sum_P W_L,P W_Rk,P^T, and J_k = sum_P gamma_k[P] B_P
-// for all densities in passes of 8 over the tensor.
+// once; gamma_k = <W_L, R_k> for 8 densities per pass over W_L; the R_k are half-transformed G at a
+// time in one launch (column blocks of the output) and K_k = sum_P W_L,P W_Rk,P^T for a stack of
+// them per rank-k launch (column blocks again); J_k = sum_P gamma_k[P] B_P for all densities in
+// passes of 8 over the tensor.
 // ================================================================================================
 enum { WS_MU_LT = 200000, WS_MU_WL, WS_MU_RT, WS_MU_WR, WS_MU_DF, WS_MU_GP, WS_MU_GAM, WS_MU_KP, WS_MU_JP,
        WS_MU_IN_L, WS_MU_IN_R, WS_MU_OUT_J, WS_MU_OUT_K };
 
-// jpart[split][k][e] = sum over the split's aux rows of gamma[k][P] Bt[P][e] for nk <= NK densities,
-// over the upper-triangle vectors (tri) or every vector of the row.
-template <typename TB, int NK>
+// gt[P][k] = gamma[k][P] for k < nk, 0 for nk <= k < NK: the NK weights of one aux row side by side.
+template <int NK>
+__global__ void k_gamma_rows(const double* gamma, size_t gstride, int nk, size_t naux, double* gt) {
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < naux * NK; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t P = idx / NK;
+    const int k = (int)(idx % NK);
+    gt[idx] = k < nk ? gamma[k * gstride + P] : 0.0;
+  }
+}
+
+// jpart[split][k][e] = sum over the split's aux rows of gt[P][k] Bt[P][e] for NK densities (gt from
+// k_gamma_rows; columns past the densities in use are zero and their sums are not stored), over the
+// upper-triangle vectors (tri) or every vector of the row. Each thread owns one 16-byte vector. The
+// weights of up to CH aux rows at a time are staged in the LDS (read as broadcasts); the tensor rows
+// go through two register buffers of PU rows: the next PU rows are loaded (row index clamped to the
+// chunk, so every load is unconditional and in bounds) while the current PU rows are used, about
+// eight fp64 multiply-adds per loaded value. The sums run over P in order.
+template <typename TB, int NK, int PU>
 __global__ __launch_bounds__(256) void k_coulomb_multi(const TB* B, size_t rowlen, size_t ld, size_t nvec, int tri,
-                                                       size_t naux, const double* gamma, size_t gstride, int nk,
-                                                       size_t pps, double* jpart) {
+                                                       size_t naux, const double* gt, int nk, size_t pps,
+                                                       double* jpart) {
   using V = Vec16<TB>;
-  constexpr int VN = V::N;
+  using raw = typename V::raw_t;
+  constexpr int VN = V::N, CH = 256;
+  __shared__ double gsh[CH * NK];
   const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nvec) return;  // no barriers below
+  const bool live = v < nvec;  // threads past the last vector load a valid one and store nothing
+  const size_t vv = live ? v : nvec - 1;
   const size_t split = blockIdx.y;
   const size_t pb = split * pps, pe = std::min(naux, pb + pps);
-  const size_t e = tri ? tri_offset(v, ld / VN, VN, ld) : v * VN;
+  const size_t e = tri ? tri_offset(vv, ld / VN, VN, ld) : vv * VN;
+  const TB* col = B + e;
   double acc[NK][VN];
 #pragma unroll
   for (int k = 0; k < NK; ++k)
 #pragma unroll
     for (int q = 0; q < VN; ++q) acc[k][q] = 0.0;
-  for (size_t P = pb; P < pe; ++P) {
-    double bv[VN];
-    V::load_nt(B + P * rowlen + e, bv);
+  for (size_t c0 = pb; c0 < pe; c0 += CH) {
+    const size_t c1 = std::min(pe, c0 + CH);
+    __syncthreads();  // the previous chunk's weights are no longer read
+    for (size_t i = threadIdx.x; i < (c1 - c0) * NK; i += 256) gsh[i] = gt[c0 * NK + i];
+    __syncthreads();
+    auto fetch = [&](size_t P, raw(&buf)[PU]) {
 #pragma unroll
-    for (int k = 0; k < NK; ++k) {
-      if (k < nk) {
-        const double g = gamma[k * gstride + P];
+      for (int h = 0; h < PU; ++h) buf[h] = V::raw_nt(col + std::min(P + h, c1 - 1) * rowlen);
+    };
+    auto use = [&](size_t P, const raw(&buf)[PU]) {
 #pragma unroll
-        for (int q = 0; q < VN; ++q) acc[k][q] += g * bv[q];
+      for (int h = 0; h < PU; ++h) {
+        if (P + h < c1) {
+          const double* g = gsh + (P + h - c0) * NK;
+          double w[VN];
+          V::widen(buf[h], w);
+#pragma unroll
+          for (int k = 0; k < NK; ++k) {
+            const double gk = g[k];
+#pragma unroll
+            for (int q = 0; q < VN; ++q) acc[k][q] += gk * w[q];
+          }
+        }
       }
+    };
+    raw b0[PU], b1[PU];
+    fetch(c0, b0);
+    for (size_t P = c0; P < c1; P += 2 * PU) {
+      fetch(P + PU, b1);
+      use(P, b0);
+      fetch(P + 2 * PU, b0);
+      use(P + PU, b1);
     }
   }
+  if (!live) return;
   double* out = jpart + split * (size_t)NK * rowlen;
 #pragma unroll
   for (int k = 0; k < NK; ++k)
@@ -2129,16 +2262,41 @@ __global__ __launch_bounds__(256) void k_coulomb_multi(const TB* B, size_t rowle
     }
 }
 
-// J_k[mu][nu] = sum_s jpart[s][k][.] at (min, max) (tri) or (nu, mu) of the transposed layout.
-__global__ void k_multi_jasm(const double* jpart, int nsplit, size_t sstride, size_t rowlen, size_t ld, size_t n, int nk,
-                             int tri, double* J) {
-  const size_t nn = n * n, total = (size_t)nk * nn;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-    const size_t k = idx / nn, rem = idx % nn, mu = rem / n, nu = rem % n;
-    const size_t src = tri ? (mu < nu ? mu * ld + nu : nu * ld + mu) : nu * ld + mu;
+// J_k[mu][nu] = sum_s jpart[s][k][.] at (min, max) (tri) or (nu, mu) of the transposed layout, in
+// 32 x 32 tiles of the stored layout (workgroup x: tile, y: density): the split sums are read along
+// stored rows and written through the LDS both as they lie and transposed, so every global access
+// runs along a row. Each value sums the splits in order.
+__global__ __launch_bounds__(256) void k_multi_jasm(const double* jpart, int nsplit, size_t sstride, size_t rowlen,
+                                                    size_t ld, size_t n, int tri, double* J) {
+  __shared__ double T[32][33];
+  const size_t nb = (n + 31) / 32;
+  const size_t br = blockIdx.x / nb, bc = blockIdx.x % nb;  // stored rows br*32.., columns bc*32..
+  if (tri && br > bc) return;  // uniform per workgroup, before any barrier
+  const double* src = jpart + blockIdx.y * rowlen;
+  double* Jk = J + blockIdx.y * n * n;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const bool diag = tri && br == bc;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int a = ty + 8 * j;
+    const size_t r = br * 32 + a, c = bc * 32 + tx;
     double s = 0.0;
-    for (int z = 0; z < nsplit; ++z) s += jpart[z * sstride + k * rowlen + src];
-    J[idx] = s;
+    if (r < n && c < n && (!diag || a <= tx))
+      for (int z = 0; z < nsplit; ++z) s += src[z * sstride + r * ld + c];
+    T[a][tx] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int a = ty + 8 * j;
+    if (tri) {  // as stored: J[r][c] (the lower half of a diagonal tile from its upper half)
+      const size_t r = br * 32 + a, c = bc * 32 + tx;
+      if (r < n && c < n) Jk[r * n + c] = (diag && a > tx) ? T[tx][a] : T[a][tx];
+    }
+    if (!diag) {  // transposed: J[c][r] = stored (r, c)
+      const size_t row = bc * 32 + a, c = br * 32 + tx;
+      if (row < n && c < n) Jk[row * n + c] = T[tx][a];
+    }
   }
 }
 
@@ -2168,52 +2326,69 @@ void multi_jk(Context& c, const Tensor& t, int prec, const double* dL, const dou
     gemm_auto(c, sp, g);
   };
   half(lt, rp, WL, 0, 0);
-  TI* rt = static_cast<TI*>(wsbuf(c, WS_MU_RT, G * rp * ld * sizeof(TI)));
-  TI* WR = static_cast<TI*>(wsbuf(c, WS_MU_WR, G * n * kdim * sizeof(TI)));
-  const int slab = prec == P64 ? 16 : 32;
-  const int ns = gemm2_splits(n, kdim, slab);
-  const size_t kper = roundup((kdim + ns - 1) / ns, slab);
-  const int nsplit = (int)((kdim + kper - 1) / kper);
-  double* kpart = dK ? static_cast<double*>(wsbuf(c, WS_MU_KP, (size_t)nsplit * n * n * sizeof(double))) : nullptr;
   double* gam = dJ ? static_cast<double*>(wsbuf(c, WS_MU_GAM, nd * naux * sizeof(double))) : nullptr;
-  double* dfac = dJ ? static_cast<double*>(wsbuf(c, WS_MU_DF, n * rp * sizeof(double))) : nullptr;
-  for (size_t g0 = 0; g0 < nd; g0 += G) {
-    const size_t gs = std::min(G, nd - g0);
-    for (size_t j = 0; j < gs; ++j)
-      hipLaunchKernelGGL((k_pack<TI>), dim3(grid_for(rp * ld)), dim3(256), 0, c.stream, dR + (g0 + j) * n * r, r, 1, rp, r,
-                         n, ld, 1.0, rt + j * rp * ld);
-    RR_HIP(hipGetLastError());
-    half(rt, gs * rp, WR, (int)rp, (long long)(n * kdim));
-    for (size_t j = 0; j < gs; ++j) {
-      const size_t k = g0 + j;
-      const TI* wr = WR + j * n * kdim;
-      if (dJ) {
-        PhaseTimer pt(c, PH_GAMMA);
-        hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(n * rp)), dim3(256), 0, c.stream, dR + k * n * r, r, 0, n, n, r, rp,
-                           1.0, dfac);
-        gamma_from_w<TI>(c, WL, nullptr, kdim, rp, n, naux, dfac, rp, gam + k * naux);
-      }
-      if (dK) {
-        {
-          PhaseTimer pt(c, PH_GEMM2);
-          GemmArgs g;
-          g.A = WL, g.lda = kdim;
-          g.B = wr, g.ldb = kdim;
-          g.C = kpart, g.ldcm = n, g.ldcn = 1, g.sCz = n * n;
-          g.M = (int)n, g.N = (int)n, g.K = (int)kdim, g.kper = (int)kper, g.nsplit = nsplit;
-          GemmSpec sp;
-          sp.prec = prec, sp.out = P64, sp.cfg = gemm2_cfg(c, n);
-          sp.lds = gemm2_lds(c, n);
-          gemm_auto(c, sp, g);
-        }
-        hipLaunchKernelGGL(k_reduce_parts, dim3(grid_for(n * n)), dim3(256), 0, c.stream, kpart, nsplit, n * n, n * n,
-                           dK + k * n * n);
+  if (dJ) {
+    // gamma_k = <W_L, R_k> for up to 8 densities per pass over W_L
+    constexpr size_t GD = 8;
+    double* dfac = static_cast<double*>(wsbuf(c, WS_MU_DF, GD * n * rp * sizeof(double)));
+    for (size_t k0 = 0; k0 < nd; k0 += GD) {
+      const size_t nk = std::min(GD, nd - k0);
+      PhaseTimer pt(c, PH_GAMMA);
+      for (size_t j = 0; j < nk; ++j)
+        hipLaunchKernelGGL((k_pack<double>), dim3(grid_for(n * rp)), dim3(256), 0, c.stream, dR + (k0 + j) * n * r, r, 0, n,
+                           n, r, rp, 1.0, dfac + j * n * rp);
+      RR_HIP(hipGetLastError());
+      gamma_from_w_multi<TI>(c, WL, kdim, rp, n, naux, dfac, (int)nk, gam + k0 * naux);
+    }
+  }
+  if (dK) {
+    const int slab = prec == P64 ? 16 : 32;
+    const int ns = gemm2_splits(n, kdim, slab);
+    const size_t kper = roundup((kdim + ns - 1) / ns, slab);
+    const int nsplit = (int)((kdim + kper - 1) / kper);
+    // K_k = sum_P W_L,P W_Rk,P^T for GR densities per launch: their W_R blocks lie stacked in WR
+    // (G per half-transform launch), and the product writes each K_k as a column block. Same
+    // splits as the single-density build, so every K_k is summed in its order. Up to 16 densities
+    // while the stacked factors stay within 8 GiB and the partial outputs within 4 GiB: the W_L
+    // panels are read once for all of them, and an orbital count that is not a multiple of the
+    // tile edge wastes tile area in one dimension only.
+    const size_t wr_one = n * kdim * sizeof(TI), kp_one = (size_t)nsplit * n * n * sizeof(double);
+    size_t GR = G;
+    while (GR + G <= std::min<size_t>(nd, 16) && (GR + G) * wr_one <= (8ull << 30) && (GR + G) * kp_one <= (4ull << 30))
+      GR += G;
+    TI* rt = static_cast<TI*>(wsbuf(c, WS_MU_RT, G * rp * ld * sizeof(TI)));
+    TI* WR = static_cast<TI*>(wsbuf(c, WS_MU_WR, GR * wr_one));
+    double* kpart = static_cast<double*>(wsbuf(c, WS_MU_KP, GR * kp_one));
+    for (size_t g0 = 0; g0 < nd; g0 += GR) {
+      const size_t gr = std::min(GR, nd - g0);
+      for (size_t h0 = 0; h0 < gr; h0 += G) {
+        const size_t gs = std::min(G, gr - h0);
+        for (size_t j = 0; j < gs; ++j)
+          hipLaunchKernelGGL((k_pack<TI>), dim3(grid_for(rp * ld)), dim3(256), 0, c.stream, dR + (g0 + h0 + j) * n * r, r, 1,
+                             rp, r, n, ld, 1.0, rt + j * rp * ld);
         RR_HIP(hipGetLastError());
+        half(rt, gs * rp, WR + h0 * n * kdim, (int)rp, (long long)(n * kdim));
       }
+      {
+        PhaseTimer pt(c, PH_GEMM2);
+        GemmArgs g;
+        g.A = WL, g.lda = kdim;
+        g.B = WR, g.ldb = kdim;
+        g.C = kpart, g.ldcm = n, g.ldcn = 1, g.sCz = gr * n * n;
+        g.nblk = (int)n, g.sCnb = (long long)(n * n);
+        g.M = (int)n, g.N = (int)(gr * n), g.K = (int)kdim, g.kper = (int)kper, g.nsplit = nsplit;
+        GemmSpec sp;
+        sp.prec = prec, sp.out = P64, sp.cfg = gemm2_cfg(c, n);
+        sp.lds = gemm2_lds(c, n);
+        gemm_auto(c, sp, g);
+      }
+      hipLaunchKernelGGL(k_reduce_parts, dim3(grid_for(gr * n * n)), dim3(256), 0, c.stream, kpart, nsplit, gr * n * n,
+                         gr * n * n, dK + g0 * n * n);
+      RR_HIP(hipGetLastError());
     }
   }
   if (dJ) {
-    constexpr int NK = 8;
+    constexpr int NK = 8, PU = 4;
     const TI* B = static_cast<const TI*>(tensor_data(t, prec));
     const size_t vn = 16 / sizeof(TI), rowlen = n * ld;
     const size_t nvec = tri ? tri_vectors(n, ld, vn) : rowlen / vn;
@@ -2223,16 +2398,20 @@ void multi_jk(Context& c, const Tensor& t, int prec, const double* dL, const dou
     const size_t pps = (naux + js - 1) / js;
     js = (naux + pps - 1) / pps;
     double* jp = static_cast<double*>(wsbuf(c, WS_MU_JP, js * NK * rowlen * sizeof(double)));
+    double* gt = static_cast<double*>(wsbuf(c, WS_MU_GP, naux * NK * sizeof(double)));
+    const size_t nb = (n + 31) / 32;
     for (size_t k0 = 0; k0 < nd; k0 += NK) {
       const int nk = (int)std::min<size_t>(NK, nd - k0);
       {
         PhaseTimer pt(c, PH_COUL);
-        hipLaunchKernelGGL((k_coulomb_multi<TI, NK>), dim3((unsigned)nx, (unsigned)js), dim3(256), 0, c.stream, B, rowlen,
-                           ld, nvec, tri, naux, gam + k0 * naux, naux, nk, pps, jp);
+        hipLaunchKernelGGL((k_gamma_rows<NK>), dim3(grid_for(naux * NK)), dim3(256), 0, c.stream, gam + k0 * naux, naux, nk,
+                           naux, gt);
+        hipLaunchKernelGGL((k_coulomb_multi<TI, NK, PU>), dim3((unsigned)nx, (unsigned)js), dim3(256), 0, c.stream, B,
+                           rowlen, ld, nvec, tri, naux, gt, nk, pps, jp);
         RR_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(k_multi_jasm, dim3(grid_for((size_t)nk * n * n)), dim3(256), 0, c.stream, jp, (int)js,
-                         NK * rowlen, rowlen, ld, n, nk, tri, dJ + k0 * n * n);
+      hipLaunchKernelGGL(k_multi_jasm, dim3((unsigned)(nb * nb), (unsigned)nk), dim3(256), 0, c.stream, jp, (int)js,
+                         NK * rowlen, rowlen, ld, n, tri, dJ + k0 * n * n);
       RR_HIP(hipGetLastError());
     }
   }
