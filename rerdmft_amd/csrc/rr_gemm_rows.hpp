@@ -252,7 +252,7 @@ inline bool rankk_kernel_fits(const GemmArgs& g) {
 
 // Split-K rank-k update C_z[m][n] = sum_{k in split z} A[m][k] B[n][k] (bf16 operands, fp32 matrix-core
 // sums, TO partial outputs): the exchange step K = W_L W_R^T over the flat (aux row, column) index;
-// with column blocks (nblk) one launch writes the K of several stacked W_R that share W_L.
+// with column blocks (COLBLK, nblk > 0) one launch writes the K of several stacked W_R that share W_L.
 // Same building blocks as gemm_rows_bf16_kernel: one persistent workgroup of eight waves per CU,
 // 256 x 256 tiles (waves of 128 x 64: eight A and four B fragments for 32 matrix-core steps per
 // stage), direct-to-LDS stages, DMA / fragment reads / matrix-core work interleaved, swapped
@@ -260,7 +260,7 @@ inline bool rankk_kernel_fits(const GemmArgs& g) {
 // the workgroups of one XCD take whole splits: the A and B panels of a split are then read through
 // that XCD's L2 once for all its tiles. The k order per output equals the other NT kernels' (32-k
 // slabs in increasing k inside the split), so partial outputs are bitwise equal to theirs.
-template <typename TO, int NST>
+template <typename TO, int NST, bool COLBLK>
 __global__ __launch_bounds__(512) void gemm_rankk_bf16_kernel(GemmArgs g) {
   constexpr int BM = 256, BN = 256, BK = 32, ROWB = 64, NW = 8;
   constexpr int TI = (BM + BN) / 16;
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(512) void gemm_rankk_bf16_kernel(GemmArgs g) {
         const int n0 = otn * BN + wn0 + 16 * j + 4 * grp;
         if (n0 >= g.N) continue;
         long long col = n0;
-        if (g.nblk > 0) {
+        if constexpr (COLBLK) {
           const int blk = n0 / g.nblk;
           col += static_cast<long long>(blk) * (g.sCnb - g.nblk);
         }

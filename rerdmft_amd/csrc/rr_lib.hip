@@ -705,7 +705,11 @@ void launch_lds(int v, const GemmArgs& g, int grid, hipStream_t s) {
           case 39: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 25, 4>), dim3(grid), dim3(512), 0, s, g); return;
           case 40: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 26, 4>), dim3(grid), dim3(512), 0, s, g); return;
           case 41: hipLaunchKernelGGL((gemm_rows_bf16_kernel<TO, 28, 4>), dim3(grid), dim3(512), 0, s, g); return;
-          case 42: hipLaunchKernelGGL((gemm_rankk_bf16_kernel<TO, 4>), dim3(grid), dim3(512), 0, s, g); return;
+          case 42:
+            // column blocks in their own instance: the plain store path keeps the plain kernel's registers
+            if (g.nblk > 0) hipLaunchKernelGGL((gemm_rankk_bf16_kernel<TO, 4, true>), dim3(grid), dim3(512), 0, s, g);
+            else hipLaunchKernelGGL((gemm_rankk_bf16_kernel<TO, 4, false>), dim3(grid), dim3(512), 0, s, g);
+            return;
           default:
             if (launch_dma_agpr(v, std::is_same<TO, double>::value, g, grid, s)) return;
             break;
@@ -1067,6 +1071,24 @@ void dgemm_small(Context& c, bool cplx, char opa, char opb, size_t m, size_t n, 
     return;
   }
   run_gemm(c, sp, g);
+}
+
+// Real row-major D = alpha op(A) op(B) + beta C through the library GEMM on the column-major views
+// (D^T = op(B)^T op(A)^T), one launch per product; C may be null (beta = 0) and D may differ from C.
+// The library's atomics mode stays off (its default), so the summation order is fixed run to run.
+// For the chains of small products (n_occ x n_occ) of the orbital alignment, where the packed
+// kernels' extra launches cost more than the products.
+void dgemm_lib(Context& c, char opa, char opb, size_t m, size_t n, size_t k, const double* A, size_t lda,
+               const double* B, size_t ldb, double alpha, const double* C, size_t ldc, double beta, double* D,
+               size_t ldd) {
+  if (m == 0 || n == 0) return;
+  auto rop = [](char o) { return o == 'N' ? rocblas_operation_none : rocblas_operation_transpose; };
+  const rocblas_status st = rocblas_gemm_ex(
+      c.blas, rop(opb), rop(opa), (rocblas_int)n, (rocblas_int)m, (rocblas_int)k, &alpha, B, rocblas_datatype_f64_r,
+      (rocblas_int)ldb, A, rocblas_datatype_f64_r, (rocblas_int)lda, &beta, C ? C : D, rocblas_datatype_f64_r,
+      (rocblas_int)(C ? ldc : ldd), D, rocblas_datatype_f64_r, (rocblas_int)ldd, rocblas_datatype_f64_r,
+      rocblas_gemm_algo_standard, 0, 0);
+  RR_REQUIRE(st == rocblas_status_success, RR_ERR_SOLVER, "rocblas gemm_ex failed");
 }
 
 // Symmetric / Hermitian eigensolve in place (row-major). On return, row j of `a` holds eigenvector
@@ -2030,49 +2052,79 @@ __global__ void k_axpby(size_t count, double a, const double* x, double b, const
     out[i] = a * x[i] + b * y[i];
 }
 // out = 1.5 X - 0.5 X (X^T X) helper pieces: Y = 1.5*X - 0.5*Z
-// out[0] = b = max_i sum_j |a_ij| of an m x m matrix, out[1] = max_ij |a_ij / b - delta_ij| (single
-// workgroup of 256 threads)
-__global__ void k_align_scale(const double* a, size_t m, double* out) {
-  __shared__ double sh[256];
-  double e = 0.0;
-  for (size_t i = threadIdx.x; i < m; i += blockDim.x) {
-    double r = 0.0;
-    for (size_t j = 0; j < m; ++j) r += fabs(a[i * m + j]);
-    e = fmax(e, r);
+// max over i of |a_i - delta_i| for the m x m elements, four independent loads in flight per thread
+__device__ inline double max_dev_identity(const double* a, size_t m, double scale) {
+  const size_t mm = m * m, step = blockDim.x;
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+  size_t i = threadIdx.x;
+  for (; i + 3 * step < mm; i += 4 * step) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t j = i + u * step;
+      d[u] = fmax(d[u], fabs(a[j] * scale - ((j / m) == (j % m) ? 1.0 : 0.0)));
+    }
   }
-  sh[threadIdx.x] = e;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-    __syncthreads();
-  }
-  const double b = sh[0];
-  __syncthreads();
-  double d = 0.0;
-  for (size_t i = threadIdx.x; i < m * m; i += blockDim.x) d = fmax(d, fabs(a[i] / b - ((i / m) == (i % m) ? 1.0 : 0.0)));
-  sh[threadIdx.x] = d;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = b, out[1] = sh[0];
+  for (; i < mm; i += step) d[0] = fmax(d[0], fabs(a[i] * scale - ((i / m) == (i % m) ? 1.0 : 0.0)));
+  return fmax(fmax(d[0], d[1]), fmax(d[2], d[3]));
 }
-// ||X^T X - I||_max into *err (single workgroup)
-__global__ void k_orth_err(const double* xtx, size_t m, double* err) {
-  __shared__ double sh[256];
-  double e = 0.0;
-  for (size_t i = threadIdx.x; i < m * m; i += blockDim.x) {
-    const double v = xtx[i] - ((i / m) == (i % m) ? 1.0 : 0.0);
-    e = fmax(e, fabs(v));
-  }
-  sh[threadIdx.x] = e;
+__device__ inline double block_max(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, sh[w]);
+  return r;
+}
+// For the symmetric m x m matrix a = X^T X (one workgroup, 1024 threads): out[0] = b = max_i sum_j |a_ij|
+// (summed along columns, which the symmetry makes the row sums, so the loads run along rows),
+// out[1] = max_ij |a_ij / b - delta_ij|.
+__global__ __launch_bounds__(1024) void k_align_scale(const double* a, size_t m, double* out) {
+  __shared__ double sh[16];
+  __shared__ double part[1024];
+  double e = 0.0;
+  if (m <= blockDim.x) {
+    // G thread groups per column, group g summing the rows i = g mod G; the G partial sums of a
+    // column are then added in group order
+    const size_t G = blockDim.x / m, g = threadIdx.x / m, j = threadIdx.x % m;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g < G) {
+      size_t i = g;
+      for (; i + 3 * G < m; i += 4 * G) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] += fabs(a[(i + u * G) * m + j]);
+      }
+      for (; i < m; i += G) r[0] += fabs(a[i * m + j]);
+    }
+    part[threadIdx.x] = (r[0] + r[1]) + (r[2] + r[3]);
     __syncthreads();
+    if (threadIdx.x < m) {
+      double cs = 0.0;
+      for (size_t h = 0; h < G; ++h) cs += part[h * m + threadIdx.x];
+      e = cs;
+    }
+  } else {
+    for (size_t j = threadIdx.x; j < m; j += blockDim.x) {
+      double r[4] = {0.0, 0.0, 0.0, 0.0};
+      size_t i = 0;
+      for (; i + 3 < m; i += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] += fabs(a[(i + u) * m + j]);
+      }
+      for (; i < m; ++i) r[0] += fabs(a[i * m + j]);
+      e = fmax(e, (r[0] + r[1]) + (r[2] + r[3]));
+    }
   }
-  if (threadIdx.x == 0) *err = sh[0];
+  const double b = block_max(e, sh);
+  const double d = block_max(max_dev_identity(a, m, 1.0 / b), sh);
+  if (threadIdx.x == 0) out[0] = b, out[1] = d;
+}
+// ||X^T X - I||_max into *err (one workgroup, 1024 threads)
+__global__ __launch_bounds__(1024) void k_orth_err(const double* xtx, size_t m, double* err) {
+  __shared__ double sh[16];
+  const double e = block_max(max_dev_identity(xtx, m, 1.0), sh);
+  if (threadIdx.x == 0) *err = e;
 }
 
 // Incremental build for D = 2 C C^T between two orbital sets (rr_ri_jk_delta):
@@ -2090,16 +2142,17 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
   double* z = static_cast<double*>(wsbuf(c, WS_T2, no * no * sizeof(double)));
   double* y = static_cast<double*>(wsbuf(c, WS_T3, no * no * sizeof(double)));
   double* err = static_cast<double*>(wsbuf(c, WS_TMP8 + 100, 2 * sizeof(double)));
-  dgemm_small(c, false, 'T', 'N', no, no, n, dA, no, dB, no, q, no);  // C_new^T C_old
+  // C_new^T C_old (k = n: the split-k product fills the device better than one library launch)
+  dgemm_small(c, false, 'T', 'N', no, no, n, dA, no, dB, no, q, no);
   // The polar factor is invariant to a positive scale; dividing by an upper bound of the largest
   // singular value puts every singular value in (0, 1], where the Newton-Schulz step X (3 - X^T X) / 2
   // converges to it (the orbitals need not be orthonormal in the plain metric, e.g. C^T S C = I, so
   // the singular values of C_new^T C_old are not bounded by 1 themselves). The bound is
   // sigma_max^2 <= ||X^T X||_inf (max row sum of |X^T X|): exact for an orthogonal X, whose alignment
   // then needs no step at all (the Frobenius norm, sqrt(n_occ) times too large there, cost about ten).
-  dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
+  dgemm_lib(c, 'T', 'N', no, no, no, q, no, q, no, 1.0, nullptr, 0, 0.0, z, no);  // X^T X
   // bound and the orthogonality error of the scaled X in one pass and one read-back
-  hipLaunchKernelGGL(k_align_scale, dim3(1), dim3(256), 0, c.stream, z, no, err);
+  hipLaunchKernelGGL(k_align_scale, dim3(1), dim3(1024), 0, c.stream, z, no, err);
   double be[2] = {0.0, 0.0};
   RR_HIP(hipMemcpyAsync(be, err, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   RR_HIP(hipStreamSynchronize(c.stream));
@@ -2112,14 +2165,14 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.0 / bnd, z, 0.0, z, z);
     aligned = false;
     for (int it = 0; it < 120 && !aligned; ++it) {
-      if (it > 0) dgemm_small(c, false, 'T', 'N', no, no, no, q, no, q, no, z, no);  // X^T X
+      if (it > 0) dgemm_lib(c, 'T', 'N', no, no, no, q, no, q, no, 1.0, nullptr, 0, 0.0, z, no);  // X^T X
       if (it == 0) {
         RR_CONTROL("align_residual", be[1]);
         const double e = be[1];
         if (!std::isfinite(e)) break;
         aligned = e < 1e-13;  // an orthogonal overlap: one step below leaves rounding level
       } else if (it % 3 == 0) {
-        hipLaunchKernelGGL(k_orth_err, dim3(1), dim3(256), 0, c.stream, z, no, err);
+        hipLaunchKernelGGL(k_orth_err, dim3(1), dim3(1024), 0, c.stream, z, no, err);
         double e = 0;
         RR_HIP(hipMemcpyAsync(&e, err, sizeof(double), hipMemcpyDeviceToHost, c.stream));
         RR_HIP(hipStreamSynchronize(c.stream));
@@ -2127,15 +2180,15 @@ void delta_build(Context& c, const Tensor& t, int prec, size_t n, size_t no, con
         if (!std::isfinite(e)) break;
         aligned = e < 1e-13;  // the step below then leaves rounding level
       }
-      dgemm_small(c, false, 'N', 'N', no, no, no, q, no, z, no, y, no);  // X (X^T X)
-      hipLaunchKernelGGL(k_axpby, dim3(grid_for(no * no)), dim3(256), 0, c.stream, no * no, 1.5, q, -0.5, y, q);
+      dgemm_lib(c, 'N', 'N', no, no, no, q, no, z, no, -0.5, q, no, 1.5, y, no);  // 1.5 X - 0.5 X (X^T X)
+      std::swap(q, y);
     }
   }
   double* cp = static_cast<double*>(wsbuf(c, WS_T4, n * no * sizeof(double)));
   // C' = C_new Q; without an alignment (occupied spaces with a nearly orthogonal direction) C' = C_new:
   // the difference identity below holds for any C', the alignment only keeps dC small.
   if (aligned)
-    dgemm_small(c, false, 'N', 'N', n, no, no, dA, no, q, no, cp, no);
+    dgemm_lib(c, 'N', 'N', n, no, no, dA, no, q, no, 1.0, nullptr, 0, 0.0, cp, no);
   else
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * no)), dim3(256), 0, c.stream, n * no, 1.0, dA, 0.0, dA, cp);
   double* dc = static_cast<double*>(wsbuf(c, WS_COCC, 2 * n * no * sizeof(double)));

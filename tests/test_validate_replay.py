@@ -46,7 +46,7 @@ def test_bench_call_sequence_is_memory_safe(replay_exe):
     s = summary(out)
     assert s["violations"] == 0, err[-3000:]
     assert s["calls"] == 0 and rc == 0, out[-3000:]
-    assert s["checks"] > 10000  # every launch of the sequence went through the hooks
+    assert s["checks"] > 5000  # every launch of the sequence went through the hooks
 
 
 def test_short_spin_orbital_coefficients_are_reported(replay_exe):

@@ -411,6 +411,16 @@ rocblas_status rocblas_zgemm(rocblas_handle, rocblas_operation ta, rocblas_opera
   gemm_check(ta, tb, m, n, k, A, lda, B, ldb, C, ldc, 16, "rocblas_zgemm");
   return rocblas_status_success;
 }
+rocblas_status rocblas_gemm_ex(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m, rocblas_int n,
+                               rocblas_int k, const void*, const void* A, rocblas_datatype, rocblas_int lda, const void* B,
+                               rocblas_datatype, rocblas_int ldb, const void*, const void* C, rocblas_datatype,
+                               rocblas_int ldc, void* D, rocblas_datatype, rocblas_int ldd, rocblas_datatype,
+                               rocblas_gemm_algo, int32_t, uint32_t) {
+  // f64 operands only (the library's sole use)
+  gemm_check(ta, tb, m, n, k, A, lda, B, ldb, C, ldc, 8, "rocblas_gemm_ex C");
+  gemm_check(ta, tb, m, n, 0, A, lda, B, ldb, D, ldd, 8, "rocblas_gemm_ex D");
+  return rocblas_status_success;
+}
 rocblas_status rocblas_ztrsm(rocblas_handle, rocblas_side side, rocblas_fill, rocblas_operation, rocblas_diagonal,
                              rocblas_int m, rocblas_int n, const rocblas_double_complex*,
                              const rocblas_double_complex* A, rocblas_int lda, rocblas_double_complex* B,
