@@ -186,6 +186,7 @@ struct Vec16<double> {
   // raw 16-byte vector (widened at use, so a thread keeps many loads in flight in few registers)
   typedef d2 raw_t;
   __device__ static raw_t raw_nt(const double* p) { return __builtin_nontemporal_load(reinterpret_cast<const d2*>(p)); }
+  __device__ static raw_t raw(const double* p) { return *reinterpret_cast<const d2*>(p); }
   __device__ static void widen(raw_t v, double* o) { o[0] = v.x, o[1] = v.y; }
 };
 template <>
@@ -198,6 +199,7 @@ struct Vec16<float> {
   __device__ static void load_nt(const float* p, double* o) { widen(raw_nt(p), o); }
   typedef f4 raw_t;
   __device__ static raw_t raw_nt(const float* p) { return __builtin_nontemporal_load(reinterpret_cast<const f4*>(p)); }
+  __device__ static raw_t raw(const float* p) { return *reinterpret_cast<const f4*>(p); }
   __device__ static void widen(raw_t v, double* o) { o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w; }
 };
 template <>
@@ -210,6 +212,7 @@ struct Vec16<bf16_t> {
   __device__ static void load_nt(const bf16_t* p, double* o) { widen(raw_nt(p), o); }
   typedef unsigned int raw_t __attribute__((ext_vector_type(4)));
   __device__ static raw_t raw_nt(const bf16_t* p) { return __builtin_nontemporal_load(reinterpret_cast<const raw_t*>(p)); }
+  __device__ static raw_t raw(const bf16_t* p) { return *reinterpret_cast<const raw_t*>(p); }
   __device__ static void widen(raw_t v, double* o) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -368,11 +371,15 @@ inline size_t tri_vectors(size_t n, size_t ld, size_t VN) {
 }
 
 // k_coulomb over the upper triangle only (symmetric tensors): jpart[split][e] for the triangle
-// elements e; the rest of jpart is left unwritten.
+// elements e; the rest of jpart is left unwritten. Each thread owns U vectors (lanes past the last
+// vector load the last one and store nothing) and takes PU aux rows per step; the rows of the next
+// step are loaded (row index clamped to the split, so every load is unconditional) before the current
+// rows are used, so 2 PU U loads can be in flight. The sums run over P in order.
 template <typename TB, int U, int PU = (sizeof(TB) == 8 ? 2 : 1)>
 __device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, size_t ld, size_t nvec, size_t naux,
                                                      const double* gamma, size_t p_per_split, double* jpart, unsigned bx_, unsigned by_, unsigned gx_) {
   using V = Vec16<TB>;
+  using raw = typename V::raw_t;
   constexpr int VN = V::N;
   const size_t W = ld / VN;
   const size_t vbase = (size_t)bx_ * (256 * U) + threadIdx.x;
@@ -384,46 +391,44 @@ __device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, s
   for (int u = 0; u < U; ++u) {
     const size_t v = vbase + (size_t)u * 256;
     valid[u] = v < nvec;
-    e[u] = valid[u] ? tri_offset(v, W, VN, ld) : 0;
+    e[u] = tri_offset(valid[u] ? v : nvec - 1, W, VN, ld);
   }
   double acc[U][VN];
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int q = 0; q < VN; ++q) acc[u][q] = 0.0;
-  size_t P = pb;  // PU aux rows per iteration: PU * U loads in flight per thread (sums still in P order)
-  if constexpr (PU > 1)
-  for (; P + PU <= pe; P += PU) {
-    typename V::raw_t bv[PU][U];
+  if (pb < pe) {
+    auto fetch = [&](size_t P, raw(&b)[PU][U]) {
 #pragma unroll
-    for (int h = 0; h < PU; ++h)
+      for (int h = 0; h < PU; ++h) {
+        const TB* row = B + std::min(P + h, pe - 1) * rowlen;
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (valid[u]) bv[h][u] = V::raw_nt(B + (P + h) * rowlen + e[u]);
-#pragma unroll
-    for (int h = 0; h < PU; ++h) {
-      const double g = gamma[P + h];
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (valid[u]) {
-          double w[VN];
-          V::widen(bv[h][u], w);
-#pragma unroll
-          for (int q = 0; q < VN; ++q) acc[u][q] += g * w[q];
-        }
-    }
-  }
-  for (; P < pe; ++P) {
-    const double g = gamma[P];
-    const TB* row = B + P * rowlen;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (valid[u]) {
-        double bv[VN];
-        V::load_nt(row + e[u], bv);
-#pragma unroll
-        for (int q = 0; q < VN; ++q) acc[u][q] += g * bv[q];
+        for (int u = 0; u < U; ++u) b[h][u] = V::raw_nt(row + e[u]);
       }
+    };
+    auto use = [&](size_t P, const raw(&b)[PU][U]) {
+#pragma unroll
+      for (int h = 0; h < PU; ++h) {
+        if (P + h < pe) {
+          const double g = gamma[P + h];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            double w[VN];
+            V::widen(b[h][u], w);
+#pragma unroll
+            for (int q = 0; q < VN; ++q) acc[u][q] += g * w[q];
+          }
+        }
+      }
+    };
+    raw b0[PU][U], b1[PU][U];
+    fetch(pb, b0);
+    for (size_t P = pb; P < pe; P += 2 * PU) {
+      fetch(P + PU, b1);
+      use(P, b0);
+      fetch(P + 2 * PU, b0);
+      use(P + PU, b1);
     }
   }
   double* out = jpart + split * rowlen;
@@ -1358,47 +1363,93 @@ __global__ __launch_bounds__(256) void k_gamma_w_partial(const TW* W, const TW* 
 
 // The same partial sums as k_gamma_w_partial (identical values, same summation order per output),
 // with each thread owning one group of 8 columns i for NPP consecutive aux rows P: the 8 weights of a
-// row nu are loaded once for NPP rows of W, and every load is a 16-byte vector.
-template <typename TW, int NPP>
-__device__ __forceinline__ void k_gamma_w_blocks_body(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+// row nu are loaded once for NPP rows of W, and every load is a 16-byte vector. Rows nu are taken in
+// pairs through two register buffers: the weights of row nu are loaded first, then the W values of
+// the next row, then row nu is summed, so the in-order load counter lets the next row's values stay
+// in flight.
+template <typename TW, int NPP, bool HAS_W2>
+__device__ __forceinline__ void k_gamma_w_blocks_impl(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
                                                         size_t naux, const double* dfac, size_t lddf, size_t rows,
-                                                        double* part, unsigned bx_, unsigned by_, unsigned gx_) {
+                                                        double* part, unsigned bx_, unsigned by_) {
+  using V = Vec16<TW>;
+  using raw = typename V::raw_t;
+  constexpr int NV = 8 / V::N;  // 16-byte vectors per 8 values
+  constexpr int NW = HAS_W2 ? 2 : 1;
   const size_t g8n = rp / 8, nblk = (naux + NPP - 1) / NPP;
   const size_t t = bx_ * (size_t)blockDim.x + threadIdx.x;
   if (t >= nblk * g8n) return;
   const size_t q = t % g8n, i0 = 8 * q, P0 = (t / g8n) * NPP;
   const size_t nu0 = by_ * rows, nu1 = std::min(n, nu0 + rows);
+  size_t off[NPP];
+#pragma unroll
+  for (int p = 0; p < NPP; ++p) off[p] = std::min(P0 + p, naux - 1) * rp + i0;
   double acc[NPP];
 #pragma unroll
   for (int p = 0; p < NPP; ++p) acc[p] = 0.0;
-  for (size_t nu = nu0; nu < nu1; ++nu) {
-    double d[8], d2[8];
+  auto fetch = [&](size_t nu, raw(&b)[NW][NPP][NV]) {
+    const size_t r = nu * kdim;
+#pragma unroll
+    for (int p = 0; p < NPP; ++p)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        b[0][p][v] = V::raw(W + r + off[p] + v * V::N);
+        if constexpr (HAS_W2) b[NW - 1][p][v] = V::raw(W2 + r + off[p] + v * V::N);
+      }
+  };
+  auto weights = [&](size_t nu, double* d, double* d2) {
     const double* df = dfac + nu * lddf + i0;
 #pragma unroll
     for (int v = 0; v < 4; ++v) Vec16<double>::load(df + 2 * v, d + 2 * v);
-    if (W2) {
+    if constexpr (HAS_W2) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) Vec16<double>::load(df + rp + 2 * v, d2 + 2 * v);
     }
+  };
+  // (sums of aux rows past the last one run on its values and are not stored: no branch per lane)
+  auto use = [&](const raw(&b)[NW][NPP][NV], const double* d, const double* d2) {
 #pragma unroll
     for (int p = 0; p < NPP; ++p) {
-      if (P0 + p < naux) {
-        const size_t off = nu * kdim + (P0 + p) * rp + i0;
-        double w[8];
-        load8<TW>(W + off, w);
+      double w[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc[p] += w[e] * d[e];
-        if (W2) {
-          load8<TW>(W2 + off, w);
+      for (int v = 0; v < NV; ++v) V::widen(b[0][p][v], w + v * V::N);
 #pragma unroll
-          for (int e = 0; e < 8; ++e) acc[p] += w[e] * d2[e];
-        }
+      for (int e = 0; e < 8; ++e) acc[p] += w[e] * d[e];
+      if constexpr (HAS_W2) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) V::widen(b[NW - 1][p][v], w + v * V::N);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[p] += w[e] * d2[e];
       }
     }
+  };
+  raw b0[NW][NPP][NV], b1[NW][NPP][NV];
+  double d[8], d2[8];
+  fetch(nu0, b0);
+  size_t nu = nu0;
+  for (; nu + 1 < nu1; nu += 2) {  // pairs of rows; the scheduling barriers keep the load order
+    weights(nu, d, d2);
+    fetch(nu + 1, b1);
+    __builtin_amdgcn_sched_barrier(0);
+    use(b0, d, d2);
+    weights(nu + 1, d, d2);
+    fetch(std::min(nu + 2, nu1 - 1), b0);
+    __builtin_amdgcn_sched_barrier(0);
+    use(b1, d, d2);
+  }
+  if (nu < nu1) {  // odd row count: the last row
+    weights(nu, d, d2);
+    use(b0, d, d2);
   }
 #pragma unroll
   for (int p = 0; p < NPP; ++p)
     if (P0 + p < naux) part[by_ * (kdim / 8) + (P0 + p) * g8n + q] = acc[p];
+}
+template <typename TW, int NPP>
+__device__ __forceinline__ void k_gamma_w_blocks_body(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
+                                                        size_t naux, const double* dfac, size_t lddf, size_t rows,
+                                                        double* part, unsigned bx_, unsigned by_, unsigned gx_) {
+  if (W2) k_gamma_w_blocks_impl<TW, NPP, true>(W, W2, kdim, rp, n, naux, dfac, lddf, rows, part, bx_, by_);
+  else k_gamma_w_blocks_impl<TW, NPP, false>(W, nullptr, kdim, rp, n, naux, dfac, lddf, rows, part, bx_, by_);
 }
 template <typename TW, int NPP>
 __global__ __launch_bounds__(256) void k_gamma_w_blocks(const TW* W, const TW* W2, size_t kdim, size_t rp, size_t n,
