@@ -50,7 +50,8 @@ const char* rr_last_error(rr_handle h);
  * fp64 / fp32 operands only; 139..141 the row-merged persistent bf16 half-transform and 142 the
  * persistent split-K bf16 rank-k update, each used where the product's shape fits it; other
  * operand types and shapes keep their default).
- * Results do not depend on the choice (every element is summed in the same order). */
+ * Results do not depend on the choice (every element is summed in the same order). A change drops
+ * the captured graph of rr_ri_jk_multi, which holds the kernels chosen when it was captured. */
 int rr_set_tile_config(rr_handle h, int cfg);
 /* How the density-taking builds (rr_nonrel_fock_ri, rr_x2c_fock_ri, rr_rkb_fock_ri) find the
  * occupied set, in place of the reference's eigendecomposition of X^H (S P S) X (e.g.

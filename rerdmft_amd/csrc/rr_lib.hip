@@ -3792,6 +3792,13 @@ int rr_set_tile_config(rr_handle h, int cfg) {
     h->ctx.err = "tile configuration out of range";
     return RR_ERR_ARG;
   }
+  if (h->ctx.multi_plan.exec) {  // its graph holds the kernels of the previous configuration
+    if (hipStreamSynchronize(h->ctx.stream) != hipSuccess) {
+      h->ctx.err = "stream synchronization failed";
+      return RR_ERR_HIP;
+    }
+    rr::drop_plan(h->ctx.multi_plan);
+  }
   return RR_OK;
 }
 

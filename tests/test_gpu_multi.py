@@ -86,3 +86,29 @@ def test_multi_rejects_mismatched_dimension(ctx):
     ctx.upload(SLOT, b)
     with pytest.raises(rr.GpuError):
         ctx.ri_jk_multi(SLOT, np.zeros((17, 3)), np.zeros((2, 17, 3)))
+
+
+@pytest.mark.parametrize("n", [72, 70])
+def test_multi_stacked_rank_k_variants_agree_bitwise(ctx, n):
+    """r = 60 puts 6 densities in a half-transform launch and stacks 12 of them in one rank-k launch
+    (13 densities: two half-transforms per stack, then a partial stack of one). Every bf16 GEMM variant,
+    the persistent rank-k kernel's column-block mode included (n = 72; at n = 70 its 4-column stores
+    would straddle the blocks and the other kernels run), gives the same J and K bit for bit, equal to
+    the single-density builds up to summation order."""
+    naux, r, nd = 40, 60, 13
+    b, l, rs = _case(naux, n, r, nd, 4400 + n)
+    ctx.upload(SLOT, b, rr.COPY_BF16)
+    outs = []
+    try:
+        for cfg in [-1, 1, 2, 5, 100, 101, 102, 112, 120, 125, 130, 134, 142]:
+            ctx.set_tile_config(cfg)
+            j, k, _ = ctx.ri_jk_multi(SLOT, l, rs, precision=rr.BF16)
+            outs.append((cfg, j, k))
+    finally:
+        ctx.set_tile_config(-1)
+    for cfg, j, k in outs[1:]:
+        assert np.array_equal(k, outs[0][2]), f"K differs for configuration {cfg}"
+        assert np.array_equal(j, outs[0][1]), f"J differs for configuration {cfg}"
+    for i in range(nd):
+        js, ks = ctx.ri_jk_factor(SLOT, None, l, rs[i], precision=rr.BF16)
+        assert rel(outs[0][1][i], js) < 1e-12 and rel(outs[0][2][i], ks) < 1e-12
