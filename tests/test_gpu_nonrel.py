@@ -164,7 +164,7 @@ def test_empty_occupied_space(ctx):
     assert np.all(k == 0.0) and np.all(j == 0.0)
 
 
-@pytest.mark.parametrize("naux,n,r", [(40, 37, 7), (120, 130, 29), (64, 200, 64)])
+@pytest.mark.parametrize("naux,n,r", [(40, 37, 7), (120, 130, 29), (64, 200, 64), (24, 330, 100)])
 def test_bf16_gemm_variants_agree_bitwise(ctx, naux, n, r):
     """Every bf16 GEMM variant (register-staged, LDS-staged single/double buffered, multi-stage
     direct-to-LDS, persistent row-merged and split-K) sums each output in the same k order: results must be
