@@ -371,15 +371,11 @@ inline size_t tri_vectors(size_t n, size_t ld, size_t VN) {
 }
 
 // k_coulomb over the upper triangle only (symmetric tensors): jpart[split][e] for the triangle
-// elements e; the rest of jpart is left unwritten. Each thread owns U vectors (lanes past the last
-// vector load the last one and store nothing) and takes PU aux rows per step; the rows of the next
-// step are loaded (row index clamped to the split, so every load is unconditional) before the current
-// rows are used, so 2 PU U loads can be in flight. The sums run over P in order.
+// elements e; the rest of jpart is left unwritten.
 template <typename TB, int U, int PU = (sizeof(TB) == 8 ? 2 : 1)>
 __device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, size_t ld, size_t nvec, size_t naux,
                                                      const double* gamma, size_t p_per_split, double* jpart, unsigned bx_, unsigned by_, unsigned gx_) {
   using V = Vec16<TB>;
-  using raw = typename V::raw_t;
   constexpr int VN = V::N;
   const size_t W = ld / VN;
   const size_t vbase = (size_t)bx_ * (256 * U) + threadIdx.x;
@@ -391,44 +387,46 @@ __device__ __forceinline__ void k_coulomb_tri_body(const TB* B, size_t rowlen, s
   for (int u = 0; u < U; ++u) {
     const size_t v = vbase + (size_t)u * 256;
     valid[u] = v < nvec;
-    e[u] = tri_offset(valid[u] ? v : nvec - 1, W, VN, ld);
+    e[u] = valid[u] ? tri_offset(v, W, VN, ld) : 0;
   }
   double acc[U][VN];
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int q = 0; q < VN; ++q) acc[u][q] = 0.0;
-  if (pb < pe) {
-    auto fetch = [&](size_t P, raw(&b)[PU][U]) {
+  size_t P = pb;  // PU aux rows per iteration: PU * U loads in flight per thread (sums still in P order)
+  if constexpr (PU > 1)
+  for (; P + PU <= pe; P += PU) {
+    typename V::raw_t bv[PU][U];
 #pragma unroll
-      for (int h = 0; h < PU; ++h) {
-        const TB* row = B + std::min(P + h, pe - 1) * rowlen;
+    for (int h = 0; h < PU; ++h)
 #pragma unroll
-        for (int u = 0; u < U; ++u) b[h][u] = V::raw_nt(row + e[u]);
-      }
-    };
-    auto use = [&](size_t P, const raw(&b)[PU][U]) {
+      for (int u = 0; u < U; ++u)
+        if (valid[u]) bv[h][u] = V::raw_nt(B + (P + h) * rowlen + e[u]);
 #pragma unroll
-      for (int h = 0; h < PU; ++h) {
-        if (P + h < pe) {
-          const double g = gamma[P + h];
+    for (int h = 0; h < PU; ++h) {
+      const double g = gamma[P + h];
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            double w[VN];
-            V::widen(b[h][u], w);
+      for (int u = 0; u < U; ++u)
+        if (valid[u]) {
+          double w[VN];
+          V::widen(bv[h][u], w);
 #pragma unroll
-            for (int q = 0; q < VN; ++q) acc[u][q] += g * w[q];
-          }
+          for (int q = 0; q < VN; ++q) acc[u][q] += g * w[q];
         }
+    }
+  }
+  for (; P < pe; ++P) {
+    const double g = gamma[P];
+    const TB* row = B + P * rowlen;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (valid[u]) {
+        double bv[VN];
+        V::load_nt(row + e[u], bv);
+#pragma unroll
+        for (int q = 0; q < VN; ++q) acc[u][q] += g * bv[q];
       }
-    };
-    raw b0[PU][U], b1[PU][U];
-    fetch(pb, b0);
-    for (size_t P = pb; P < pe; P += 2 * PU) {
-      fetch(P + PU, b1);
-      use(P, b0);
-      fetch(P + 2 * PU, b0);
-      use(P + PU, b1);
     }
   }
   double* out = jpart + split * rowlen;
